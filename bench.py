@@ -1,0 +1,184 @@
+"""Benchmark: 2D MUSCL-Hancock + HLLC + MC (FP64) on Kelvin-Helmholtz 4096^2.
+
+A step is one MUSCL-Hancock update of the whole grid (CFL time step, both
+sweeps, conserved update, CFL reduction for the next step), i.e. one launch
+of the fused step kernel.  Inputs are resident in HBM before the timed region;
+W warm-up steps (including the first step, which reads the caller's primitive
+state) run first, then exactly K steps are timed with HIP events on the
+stream the kernels are launched on.
+
+Prints one JSON line.  Extra diagnostics go to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, AMD public spec
+ALGO_BYTES_PER_CELL = 64       # 4 FP64 conserved fields read once and written once
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "mh2d_step_kh4096_pmc.md")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per step launch from the committed rocprofv3 PMC summary, if any."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(n: int, t_sample: float, steps_sample: int):
+    """The reference's OpenMP build (oracle/_ref) on the same KH problem for a
+    bounded number of steps (tf = time after `steps_sample` steps); falls back
+    to the oracle's OpenMP port when the reference build cannot be loaded."""
+    import ctypes as C
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    from hydrofv_amd import abi, problems
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    try:
+        ref = abi.bind(C.CDLL(O.REF_OMP_SO))
+        kind = "reference"
+    except OSError as e:
+        log(f"cpu_baseline: reference build not loadable ({e}); timing the oracle port")
+        ref = None
+        kind = "port"
+    if ref is not None:
+        sim = problems.kelvin_helmholtz(ref, n)
+        t0 = time.perf_counter()
+        res = sim.launch("muscl_hancock_2d", "riemann_solver_hllc", "monotonized_central", 0.8, t_sample)
+        sec = time.perf_counter() - t0
+        steps = res.num_steps
+        sim.close()
+    else:
+        import hydrofv_amd
+        sim = problems.kelvin_helmholtz(hydrofv_amd.load(), n)
+        import _cases as K
+        grid = K.oracle_grid(sim, "monotonized_central", nthreads=threads)
+        sim.close()
+        t0 = time.perf_counter()
+        r = grid.run(0.8, max_steps=steps_sample)
+        sec = time.perf_counter() - t0
+        steps = r.num_steps
+    value = n * n * steps / sec
+    return {"value": value, "unit": "cell-updates/s", "cores": threads, "kind": kind,
+            "sample": f"KH {n}x{n}, {steps} steps (the 10 CFL-shrink steps), {sec:.1f} s wall, "
+                      f"OMP_NUM_THREADS={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--strip-rows", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import hydrofv_amd
+    from hydrofv_amd import gpu, problems
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    torch.cuda.init()
+    lib = hydrofv_amd.load()
+    n = args.n
+    t_setup = time.perf_counter()
+    sim = problems.kelvin_helmholtz(lib, n)
+    cfg = problems.device_config(sim, limiter=3, solver=2, cfl=0.8, strip_rows=args.strip_rows)
+    dev = gpu.MH2D(lib, cfg)
+    W = sim.primitive()
+    U = sim.conserved()
+    sim.close()
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    dev.upload(W, U, stream=sh)
+    del W, U
+    dev.start(stream=sh)                      # step 1 (from the primitive state)
+    dev.advance(max(args.warmup - 1, 0), use_graph=not args.no_graph, stream=sh)
+    st0 = dev.status(stream=sh)
+    log(f"setup+warmup {time.perf_counter() - t_setup:.1f} s; steps={st0.num_steps} t={st0.t:.6g}")
+    assert st0.err_kind == 0, gpu.ERR_NAMES.get(st0.err_kind)
+
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    dev.advance(args.steps, use_graph=not args.no_graph, stream=sh)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    st = dev.status(stream=sh)
+    assert st.err_kind == 0 and st.num_steps == st0.num_steps + args.steps, (st.err_kind, st.num_steps)
+    ms_per_step = ms / args.steps
+    cells = n * n
+    value = cells * args.steps / (ms / 1e3)
+    achieved = ALGO_BYTES_PER_CELL * cells / (ms_per_step / 1e3) / 1e9
+    log(f"{ms_per_step:.4f} ms/step, {value / 1e9:.2f} G cell-updates/s, {achieved:.0f} GB/s algorithmic")
+    dev.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sample_steps = 10
+        probe = gpu.MH2D(lib, cfg)
+        sim2 = problems.kelvin_helmholtz(lib, n)
+        probe.upload(sim2.primitive(), sim2.conserved())
+        sim2.close()
+        probe.start()
+        probe.advance(sample_steps - 1, use_graph=False)
+        t_sample = probe.status().t
+        probe.close()
+        try:
+            cpu = cpu_baseline(n, t_sample, sample_steps)
+            log(f"cpu baseline: {cpu}")
+        except Exception as e:  # the GPU figure stands without it
+            log(f"cpu baseline failed: {e!r}")
+
+    traffic = measured_traffic()
+    line = {
+        "metric": "cell-updates/s FP64 MUSCL-HLLC 4096²; % HBM roofline; batched & adjoint ratio",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "kelvin_helmholtz_2d", "cells": f"{n}x{n}", "ghost_cells": 2, "gamma": "5/3",
+                   "boundary": "periodic", "limiter": "monotonized_central", "riemann_solver": "hllc",
+                   "cfl": 0.8, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

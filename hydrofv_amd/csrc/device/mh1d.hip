@@ -1,0 +1,389 @@
+// mh1d.hip -- 1D MUSCL-Hancock run (muscl_hancock_1d,
+// integrator_muscl_hancock.c:26-383) as one resident workgroup.
+//
+// A shock tube of ~10^3 cells cannot fill the chip, and its time loop is a
+// sequence of ~500 dependent steps each made of four grid-wide phases
+// (reconstruct, flux, update, c2p + boundary fill [+ geometric source]).
+// One workgroup of 1024 lanes runs the whole loop with __syncthreads()
+// between phases: no per-step launch or host round trip, and the CFL maximum
+// is a workgroup reduction.  The arrays live in global memory (L2-resident
+// at these sizes), so any grid length is accepted.
+#include <hip/hip_runtime.h>
+
+#include "hfv_ctrl.hpp"
+#include "hfv_math.hpp"
+#include "hfv_gpu_internal.hpp"
+
+namespace hfv {
+
+constexpr int T1D = 1024;
+
+struct Grid1D {
+    int n, g, tx;
+    int coord;  // 1 cartesian, 4 cylindrical, 5 spherical
+    double gamma, tol;
+    int bc[2];
+};
+
+struct Arrays1D {
+    double* w[3];
+    double* u[3];
+    double* lo[3];
+    double* hi[3];
+    double* fl[3];
+    const double* area;
+    const double* mid;
+    const double* vol;
+};
+
+// boundary.c:411-511 (x_min first, then x_max, per ghost layer)
+__device__ void bc_1d(const Grid1D& G, double* const* w, int t)
+{
+    const int g = G.g, n = G.n;
+    for (int i = t; i < g; i += T1D) {
+        if (G.bc[0] == BC_REFLECTIVE || G.bc[0] == BC_TRANSMISSIVE) {
+            w[0][g - 1 - i] = w[0][g + i];
+            w[1][g - 1 - i] = (G.bc[0] == BC_REFLECTIVE) ? -w[1][g + i] : w[1][g + i];
+            w[2][g - 1 - i] = w[2][g + i];
+        }
+        if (G.bc[1] == BC_REFLECTIVE || G.bc[1] == BC_TRANSMISSIVE) {
+            w[0][g + n + i] = w[0][g + n - 1 - i];
+            w[1][g + n + i] = (G.bc[1] == BC_REFLECTIVE) ? -w[1][g + n - 1 - i] : w[1][g + n - 1 - i];
+            w[2][g + n + i] = w[2][g + n - 1 - i];
+        } else if (G.bc[1] == BC_PERIODIC) {
+            for (int f = 0; f < 3; f++) {
+                w[f][g + n + i] = w[f][g + i];
+                w[f][g - 1 - i] = w[f][g + n - 1 - i];
+            }
+        }
+    }
+}
+
+__device__ void c2p_all(const Grid1D& G, const Arrays1D& A, int t)
+{
+    for (int k = t; k < G.tx; k += T1D) {
+        double w[3];
+        c2p_1d(G.gamma, A.vol[k], A.u[0][k], A.u[1][k], A.u[2][k], w);
+        A.w[0][k] = w[0];
+        A.w[1][k] = w[1];
+        A.w[2][k] = w[2];
+    }
+}
+
+// source_term.c:28-43
+__device__ __forceinline__ void geo_rate(double pre, double gamma, double rho, double sm, double se, double* o)
+{
+    const double pr = (gamma - 1) * (se - 0.5 * sm * sm / rho);
+    o[0] = pre * sm;
+    o[1] = pre * sm * sm / rho;
+    o[2] = pre * sm * (se + pr) / rho;
+}
+
+// Runs steps until the run finishes or 'budget' steps have been taken (the
+// host relaunches until finished, so no single launch is unbounded).
+template <int LIM, int SOLVER>
+__global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays1D A, Ctrl* ctrl, long long budget)
+{
+    __shared__ double red[T1D / 64];
+    const int t = threadIdx.x;
+    const int g = G.g, n = G.n;
+    const double gamma = G.gamma;
+
+    // CFL maximum of the interior (utils.c:28-53)
+    auto cfl_max = [&]() -> double {
+        double s = -INFINITY;
+        for (int k = g + t; k < g + n; k += T1D) {
+            const double a = sound_speed(gamma, A.w[0][k], A.w[2][k]);
+            const double v = a + fabs(A.w[1][k]);
+            if (v > s) s = v;
+        }
+        s = wave_max(s);
+        if ((t & 63) == 0) red[t >> 6] = s;
+        __syncthreads();
+        double m = red[0];
+        for (int k = 1; k < T1D / 64; k++) m = (red[k] > m) ? red[k] : m;
+        __syncthreads();
+        return m;
+    };
+
+    Ctrl c = *ctrl;
+    c.key[0] = smax_key(cfl_max());
+    for (long long taken = 0; taken < budget; taken++) {
+        int dt_zero;
+        const StepPlan plan = plan_step(&c, rp, &dt_zero);
+        if (!plan.run) {
+            if (dt_zero) c.err_kind = ERR_DT_ZERO;
+            break;
+        }
+        const double dt = plan.dt;
+        // reconstruct cells g-1 .. g+n (146-246)
+        for (int i = t; i < n + 2; i += T1D) {
+            const int k = g + i - 1;
+            const double s[3] = {A.w[0][k], A.w[1][k], A.w[2][k]};
+            const double cc = dt * A.area[k] / A.vol[k];
+            const double a = sound_speed(gamma, s[0], s[2]);
+            double d[3];
+            for (int f = 0; f < 3; f++) d[f] = limit<LIM>(s[f] - A.w[f][k - 1], A.w[f][k + 1] - s[f]);
+            double lo[3], hi[3];
+            predict_1d(s, d, cc, a, lo, hi);
+            for (int f = 0; f < 3; f++) {
+                A.lo[f][i] = lo[f];
+                A.hi[f][i] = hi[f];
+            }
+        }
+        __syncthreads();
+        // interface fluxes 0..n (252-295).  A failed flux is not reported:
+        // the reference overwrites the flux loop's error status with that of
+        // the boundary fill that follows (296-321), so the run continues.
+        for (int i = t; i <= n; i += T1D) {
+            const double l[3] = {A.hi[0][i], A.hi[1][i], A.hi[2][i]};
+            const double r[3] = {A.lo[0][i + 1], A.lo[1][i + 1], A.lo[2][i + 1]};
+            double f[3];
+            if (flux<SOLVER, 3>(gamma, G.tol, l, r, f) != ERR_NONE) {
+                f[0] = f[1] = f[2] = __builtin_nan("");
+            }
+            for (int q = 0; q < 3; q++) A.fl[q][i] = f[q] * dt;
+        }
+        __syncthreads();
+        // update in the reference's order: a cell first gains its left face's
+        // flux, then loses its right face's, both scaled by its own face-array
+        // entry (287-294)
+        for (int k = g - 1 + t; k <= g + n; k += T1D) {
+            const double ak = A.area[k];
+            for (int q = 0; q < 3; q++) {
+                double x = A.u[q][k];
+                if (k >= g) x += A.fl[q][k - g] * ak;
+                if (k < g + n) x -= A.fl[q][k - g + 1] * ak;
+                A.u[q][k] = x;
+            }
+        }
+        __syncthreads();
+        c2p_all(G, A, t);
+        __syncthreads();
+        bc_1d(G, A.w, t);
+        __syncthreads();
+        if (G.coord == 4 || G.coord == 5) {
+            // source_term.c:45-175: RK4 per interior cell, then c2p and fill
+            const double alpha = (G.coord == 4) ? 1.0 : 2.0;
+            for (int k = g + t; k < g + n; k += T1D) {
+                const double vol = A.vol[k];
+                const double m = A.u[0][k], mx = A.u[1][k], e = A.u[2][k];
+                const double pre = -alpha * vol / A.mid[k];
+                double k1[3], k2[3], k3[3], k4[3];
+                geo_rate(pre, gamma, A.w[0][k], mx / vol, e / vol, k1);
+                double tm = m + 0.5 * dt * k1[0], tu = mx + 0.5 * dt * k1[1], te = e + 0.5 * dt * k1[2];
+                geo_rate(pre, gamma, tm / vol, tu / vol, te / vol, k2);
+                tm = m + 0.5 * dt * k2[0]; tu = mx + 0.5 * dt * k2[1]; te = e + 0.5 * dt * k2[2];
+                geo_rate(pre, gamma, tm / vol, tu / vol, te / vol, k3);
+                tm = m + dt * k3[0]; tu = mx + dt * k3[1]; te = e + dt * k3[2];
+                geo_rate(pre, gamma, tm / vol, tu / vol, te / vol, k4);
+                A.u[0][k] += dt * (k1[0] + 2.0 * k2[0] + 2.0 * k3[0] + k4[0]) / 6.0;
+                A.u[1][k] += dt * (k1[1] + 2.0 * k2[1] + 2.0 * k3[1] + k4[1]) / 6.0;
+                A.u[2][k] += dt * (k1[2] + 2.0 * k2[2] + 2.0 * k3[2] + k4[2]) / 6.0;
+            }
+            __syncthreads();
+            c2p_all(G, A, t);
+            __syncthreads();
+            bc_1d(G, A.w, t);
+            __syncthreads();
+        }
+        c.t_prev = c.t;
+        c.t = plan.t + dt;
+        c.dt = dt;
+        c.num_steps = plan.n + 1;
+        c.key[0] = smax_key(cfl_max());
+    }
+    if (t == 0) *ctrl = c;
+}
+
+__global__ void k_probe_flux3(int solver, double gamma, double tol, long long n, const double* wl, const double* wr,
+                              double* f, int* err)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double l[3], r[3], o[3];
+    for (int q = 0; q < 3; q++) {
+        l[q] = wl[3 * i + q];
+        r[q] = wr[3 * i + q];
+    }
+    const int e = (solver == HLLC) ? flux<HLLC, 3>(gamma, tol, l, r, o) : flux<EXACT, 3>(gamma, tol, l, r, o);
+    for (int q = 0; q < 3; q++) f[3 * i + q] = o[q];
+    err[i] = e;
+}
+
+__global__ void k_probe_flux4(int solver, double gamma, double tol, long long n, const double* wl, const double* wr,
+                              double* f, int* err)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double l[4], r[4], o[4];
+    for (int q = 0; q < 4; q++) {
+        l[q] = wl[4 * i + q];
+        r[q] = wr[4 * i + q];
+    }
+    const int e = (solver == HLLC) ? flux<HLLC, 4>(gamma, tol, l, r, o) : flux<EXACT, 4>(gamma, tol, l, r, o);
+    for (int q = 0; q < 4; q++) f[4 * i + q] = o[q];
+    err[i] = e;
+}
+
+__global__ void k_probe_limit(int lim, long long n, const double* a, const double* b, double* out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = limit_rt(lim, a[i], b[i]);
+}
+
+}  // namespace hfv
+
+using namespace hfv;
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    ~DevBuf() { if (p) hipFree(p); }
+    bool alloc(size_t n) { return hipMalloc(&p, n * sizeof(T) + 64) == hipSuccess; }
+};
+
+}  // namespace
+
+extern "C" {
+
+int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U[3], const double* area,
+                 const double* mid, const double* volume, hfv_run_status* st, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (cfg->num_cells_x < 2 || cfg->num_ghost_cells_side < 2) return HFV_EINVAL;
+    if (cfg->slope_limiter < 1 || cfg->slope_limiter > 3) return HFV_EINVAL;
+    if (cfg->riemann_solver != HLLC && cfg->riemann_solver != EXACT) return HFV_EINVAL;
+    for (int q = 0; q < 2; q++)
+        if (cfg->bc[q] < BC_NONE || cfg->bc[q] > BC_PERIODIC) return HFV_EINVAL;
+    Grid1D G;
+    G.n = cfg->num_cells_x;
+    G.g = cfg->num_ghost_cells_side;
+    G.tx = G.n + 2 * G.g;
+    G.coord = cfg->coord_sys_flag;
+    G.gamma = cfg->gamma;
+    G.tol = cfg->tol;
+    G.bc[0] = cfg->bc[0];
+    G.bc[1] = cfg->bc[1];
+    RunParams rp;
+    rp.cfl = cfg->cfl;
+    rp.shrink_factor = cfg->cfl_initial_shrink_factor;
+    rp.shrink_steps = cfg->cfl_initial_shrink_num_steps;
+    rp.dim = 1;
+    rp.tf = cfg->tf;
+    rp.max_steps = cfg->max_steps;
+    rp.fixed_dt = cfg->fixed_dt;
+    rp.dx = cfg->dx;
+    rp.dy = 0.0;
+
+    const size_t tx = (size_t)G.tx;
+    const size_t stride = (tx + 2 + 7) / 8 * 8;
+    DevBuf<double> buf;
+    DevBuf<Ctrl> dctrl;
+    if (!buf.alloc(stride * 18) || !dctrl.alloc(1)) return HFV_ENOMEM;
+    Arrays1D A;
+    for (int f = 0; f < 3; f++) {
+        A.w[f] = buf.p + stride * f;
+        A.u[f] = buf.p + stride * (3 + f);
+        A.lo[f] = buf.p + stride * (6 + f);
+        A.hi[f] = buf.p + stride * (9 + f);
+        A.fl[f] = buf.p + stride * (12 + f);
+    }
+    double* geo = buf.p + stride * 15;
+    A.area = geo;
+    A.mid = geo + stride;
+    A.vol = geo + 2 * stride;
+    for (int f = 0; f < 3; f++) {
+        if (hipMemcpyAsync(A.w[f], W[f], tx * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) return HFV_EDEVICE;
+        if (hipMemcpyAsync(A.u[f], U[f], tx * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) return HFV_EDEVICE;
+    }
+    if (hipMemcpyAsync(geo, area, (tx + 1) * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(geo + stride, mid, tx * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(geo + 2 * stride, volume, tx * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
+        return HFV_EDEVICE;
+    Ctrl c0 = {};
+    c0.err_cell = -1;
+    if (hipMemcpyAsync(dctrl.p, &c0, sizeof(Ctrl), hipMemcpyHostToDevice, s) != hipSuccess) return HFV_EDEVICE;
+
+    const int lim = cfg->slope_limiter;
+    const bool hllc = cfg->riemann_solver == HLLC;
+    const long long budget = 1LL << 16;  // steps per launch
+    Ctrl c;
+    for (;;) {
+#define HFV_L1D(L, S) hipLaunchKernelGGL((k_mh1d_run<L, S>), dim3(1), dim3(T1D), 0, s, G, rp, A, dctrl.p, budget)
+        if (hllc) {
+            if (lim == MINMOD) HFV_L1D(MINMOD, HLLC);
+            else if (lim == VAN_LEER) HFV_L1D(VAN_LEER, HLLC);
+            else HFV_L1D(MC, HLLC);
+        } else {
+            if (lim == MINMOD) HFV_L1D(MINMOD, EXACT);
+            else if (lim == VAN_LEER) HFV_L1D(VAN_LEER, EXACT);
+            else HFV_L1D(MC, EXACT);
+        }
+#undef HFV_L1D
+        if (hipGetLastError() != hipSuccess) return HFV_EDEVICE;
+        if (hipMemcpyAsync(&c, dctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s) != hipSuccess) return HFV_EDEVICE;
+        if (hipStreamSynchronize(s) != hipSuccess) return HFV_EDEVICE;
+        const bool done = c.err_kind != ERR_NONE || !(c.t < rp.tf) || (rp.max_steps >= 0 && c.num_steps >= rp.max_steps);
+        if (done) break;
+    }
+    for (int f = 0; f < 3; f++) {
+        if (hipMemcpyAsync(W[f], A.w[f], tx * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) return HFV_EDEVICE;
+        if (hipMemcpyAsync(U[f], A.u[f], tx * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) return HFV_EDEVICE;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return HFV_EDEVICE;
+    st->t = c.t;
+    st->dt = c.dt;
+    st->t_prev = c.t_prev;
+    st->num_steps = c.num_steps;
+    st->err_kind = c.err_kind;
+    st->err_cell = c.err_cell;
+    st->cur = 0;
+    st->smax[0] = smax_from_key(c.key[0]);
+    st->smax[1] = 0.0;
+    st->finished = 1;
+    return HFV_OK;
+}
+
+int hfv_probe_flux(int nf, int solver, double gamma, double tol, long long n, const double* wl, const double* wr,
+                   double* f, int* err)
+{
+    if ((nf != 3 && nf != 4) || n < 0) return HFV_EINVAL;
+    if (n == 0) return HFV_OK;
+    DevBuf<double> dl, dr, df;
+    DevBuf<int> de;
+    const size_t m = (size_t)n * nf;
+    if (!dl.alloc(m) || !dr.alloc(m) || !df.alloc(m) || !de.alloc((size_t)n)) return HFV_ENOMEM;
+    if (hipMemcpy(dl.p, wl, m * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dr.p, wr, m * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return HFV_EDEVICE;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (nf == 3) hipLaunchKernelGGL(k_probe_flux3, grid, block, 0, 0, solver, gamma, tol, n, dl.p, dr.p, df.p, de.p);
+    else hipLaunchKernelGGL(k_probe_flux4, grid, block, 0, 0, solver, gamma, tol, n, dl.p, dr.p, df.p, de.p);
+    if (hipGetLastError() != hipSuccess) return HFV_EDEVICE;
+    if (hipMemcpy(f, df.p, m * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(err, de.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return HFV_EDEVICE;
+    return HFV_OK;
+}
+
+int hfv_probe_limit(int limiter, long long n, const double* a, const double* b, double* out)
+{
+    if (limiter < 1 || limiter > 3 || n < 0) return HFV_EINVAL;
+    if (n == 0) return HFV_OK;
+    DevBuf<double> da, db, dout;
+    if (!da.alloc((size_t)n) || !db.alloc((size_t)n) || !dout.alloc((size_t)n)) return HFV_ENOMEM;
+    if (hipMemcpy(da.p, a, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(db.p, b, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return HFV_EDEVICE;
+    hipLaunchKernelGGL(k_probe_limit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, limiter, n, da.p, db.p,
+                       dout.p);
+    if (hipGetLastError() != hipSuccess) return HFV_EDEVICE;
+    if (hipMemcpy(out, dout.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return HFV_EDEVICE;
+    return HFV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,224 @@
+/*
+ * hfv_integrator.c -- muscl_hancock_1d / muscl_hancock_2d (integrator.h:145-176)
+ * running on the MI355X.
+ *
+ * The reference's time loop (integrator_muscl_hancock.c:124-349, 530-1160)
+ * becomes: copy the System's arrays to the device once, run the steps there
+ * (time control included, see hfv_ctrl.hpp), copy the final primitive and
+ * conserved arrays back.  SimulationStatus and the error codes/messages follow
+ * the reference.
+ */
+#include <stdio.h>
+#include <string.h>
+
+#include "hydrofv/hfv_gpu.h"
+#include "hydrofv/hydro.h"
+
+const char *hfv_device_error_string(void);
+
+static ErrorStatus device_error(int rc)
+{
+    char msg[256];
+    if (rc == HFV_ENOMEM) return WRAP_RAISE_ERROR(MEMORY_ERROR, "Failed to allocate device memory.");
+    if (rc == HFV_EINVAL) return WRAP_RAISE_ERROR(VALUE_ERROR, "Configuration not supported by the device integrator.");
+    snprintf(msg, sizeof msg, "HIP runtime error: %s", hfv_device_error_string());
+    return WRAP_RAISE_ERROR(FAILURE, msg);
+}
+
+static ErrorStatus run_error(int kind)
+{
+    switch (kind) {
+    case HFV_ERR_DT_ZERO:
+        return WRAP_RAISE_ERROR(VALUE_ERROR, "dt is zero.");
+    case HFV_ERR_HLLC_SPEED:
+        return WRAP_RAISE_ERROR(VALUE_ERROR,
+                                "Invalid wave speed for the HLLC riemann solver. Possible fix: use a smaller CFL.");
+    case HFV_ERR_EXACT_BRACKET:
+        return WRAP_RAISE_ERROR(FAILURE, "Failed to bracket the root for bisection method.");
+    case HFV_ERR_EXACT_BISECT:
+        return WRAP_RAISE_ERROR(FAILURE,
+                                "Bisection method did not converge after 5001 iterations for exact riemann solver.");
+    default:
+        return WRAP_RAISE_ERROR(UNKNOWN_ERROR, "Unknown device failure.");
+    }
+}
+
+static void fill_status(SimulationStatus *out, const hfv_run_status *st)
+{
+    out->num_steps = st->num_steps;
+    out->t = st->t;
+    out->dt = st->dt;
+    if (st->err_kind == HFV_ERR_DT_ZERO) {
+        out->dt = 0.0;
+    } else if (st->err_kind != HFV_ERR_NONE) {
+        /* the failing step is not counted (integrator_muscl_hancock.c:831-834) */
+        out->num_steps = st->num_steps - 1;
+        out->t = st->t_prev;
+    }
+}
+
+static int uniform(const double *a, long n)
+{
+    for (long k = 1; k < n; k++)
+        if (a[k] != a[0]) return 0;
+    return 1;
+}
+
+static ErrorStatus common_checks(const System *system, const BoundaryConditionParam *bc, const StoringParam *sp,
+                                 int two_d)
+{
+    if (sp->is_storing) {
+        return WRAP_RAISE_ERROR(NOT_IMPLEMENTED_ERROR, "Snapshot storing is not provided by this library.");
+    }
+    const int flags[4] = {bc->boundary_condition_flag_x_min_, bc->boundary_condition_flag_x_max_,
+                          bc->boundary_condition_flag_y_min_, bc->boundary_condition_flag_y_max_};
+    for (int k = 0; k < (two_d ? 4 : 2); k++) {
+        if (flags[k] == BOUNDARY_CONDITION_CUSTOM)
+            return WRAP_RAISE_ERROR(NOT_IMPLEMENTED_ERROR, "Custom boundary conditions are not supported on the device.");
+        if (flags[k] < BOUNDARY_CONDITION_NONE || flags[k] > BOUNDARY_CONDITION_PERIODIC)
+            return WRAP_RAISE_ERROR(VALUE_ERROR, "Boundary condition flag not recognized.");
+    }
+    const int g = system->num_ghost_cells_side;
+    if (system->num_cells_x < g || (two_d && system->num_cells_y < g))
+        return WRAP_RAISE_ERROR(VALUE_ERROR, "The device integrator needs at least as many cells per direction as ghost cells.");
+    return make_success_error_status();
+}
+
+/* Runs the 2D loop; max_steps < 0 and fixed_dt <= 0 give the reference loop. */
+ErrorStatus hfv_muscl_hancock_2d_run(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
+                                     StoringParam *sp, Settings *settings, SimulationParam *prm,
+                                     SimulationStatus *status, long long max_steps, double fixed_dt)
+{
+    (void)settings;
+    char msg[256];
+    if (system->coord_sys_flag_ != COORD_SYS_CARTESIAN_2D) {
+        snprintf(msg, sizeof msg, "Wrong coordinate system. Supported coordinate system: \"cartesian_2d\", got: \"%s\"",
+                 system->coord_sys ? system->coord_sys : "");
+        return WRAP_RAISE_ERROR(VALUE_ERROR, msg);
+    }
+    ErrorStatus e = WRAP_TRACEBACK(common_checks(system, bc, sp, 1));
+    if (e.return_code != SUCCESS) return e;
+    const int g = system->num_ghost_cells_side;
+    const long tx = system->num_cells_x + 2 * g, ty = system->num_cells_y + 2 * g;
+    if (!uniform(system->surface_area_x_, tx) || !uniform(system->surface_area_y_, ty) ||
+        !uniform(system->volume_, tx * ty)) {
+        return WRAP_RAISE_ERROR(NOT_IMPLEMENTED_ERROR, "The device integrator needs uniform Cartesian face areas and volumes.");
+    }
+
+    hfv_mh2d_config cfg;
+    memset(&cfg, 0, sizeof cfg);
+    cfg.num_cells_x = system->num_cells_x;
+    cfg.num_cells_y = system->num_cells_y;
+    cfg.num_ghost_cells_side = g;
+    cfg.gamma = system->gamma;
+    cfg.gravity = system->gravity;
+    cfg.dx = system->dx_;
+    cfg.dy = system->dy_;
+    cfg.area_x = system->surface_area_x_[0];
+    cfg.area_y = system->surface_area_y_[0];
+    cfg.volume = system->volume_[0];
+    cfg.slope_limiter = ip->slope_limiter_flag_;
+    cfg.riemann_solver = ip->riemann_solver_flag_;
+    cfg.tol = ip->tol;
+    cfg.bc[0] = bc->boundary_condition_flag_x_min_;
+    cfg.bc[1] = bc->boundary_condition_flag_x_max_;
+    cfg.bc[2] = bc->boundary_condition_flag_y_min_;
+    cfg.bc[3] = bc->boundary_condition_flag_y_max_;
+    cfg.cfl = ip->cfl;
+    cfg.cfl_initial_shrink_factor = ip->cfl_initial_shrink_factor;
+    cfg.cfl_initial_shrink_num_steps = ip->cfl_initial_shrink_num_steps;
+    cfg.tf = prm->tf;
+    cfg.max_steps = max_steps;
+    cfg.fixed_dt = fixed_dt;
+
+    status->num_steps = 0;
+    status->dt = 0.0;
+    status->t = 0.0;
+    sp->store_count_ = 0;
+
+    hfv_mh2d *h = NULL;
+    int rc = hfv_mh2d_create(&cfg, &h);
+    if (rc != HFV_OK) return device_error(rc);
+    double *W[4] = {system->density_, system->velocity_x_, system->velocity_y_, system->pressure_};
+    double *U[4] = {system->mass_, system->momentum_x_, system->momentum_y_, system->energy_};
+    hfv_run_status st;
+    rc = hfv_mh2d_upload(h, (const double *const *)W, (const double *const *)U, NULL);
+    if (rc == HFV_OK) rc = hfv_mh2d_start(h, NULL);
+    if (rc == HFV_OK && max_steps >= 0) {
+        /* known step count: enqueue every launch, then check once */
+        if (max_steps > 1) rc = hfv_mh2d_advance(h, max_steps - 1, 1, NULL);
+        if (rc == HFV_OK) rc = hfv_mh2d_status(h, &st, NULL);
+    } else if (rc == HFV_OK) {
+        /* run to tf: batches of queued launches between status checks; a
+         * launch after the end only forwards the state */
+        long long batch = 15;
+        for (;;) {
+            rc = hfv_mh2d_status(h, &st, NULL);
+            if (rc != HFV_OK || st.finished) break;
+            rc = hfv_mh2d_advance(h, batch, 1, NULL);
+            if (rc != HFV_OK) break;
+            batch = (batch < 48) ? 48 : (batch < 480 ? 2 * batch : 480);
+        }
+    }
+    if (rc == HFV_OK) rc = hfv_mh2d_download(h, W, U, NULL);
+    hfv_mh2d_destroy(h);
+    if (rc != HFV_OK) return device_error(rc);
+    fill_status(status, &st);
+    if (st.err_kind != HFV_ERR_NONE) return run_error(st.err_kind);
+    return make_success_error_status();
+}
+
+ErrorStatus hfv_muscl_hancock_1d_run(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
+                                     StoringParam *sp, Settings *settings, SimulationParam *prm,
+                                     SimulationStatus *status, long long max_steps, double fixed_dt)
+{
+    (void)settings;
+    const int f = system->coord_sys_flag_;
+    if (f != COORD_SYS_CARTESIAN_1D && f != COORD_SYS_CYLINDRICAL_1D && f != COORD_SYS_SPHERICAL_1D) {
+        return WRAP_RAISE_ERROR(VALUE_ERROR, "Wrong coordinate system, expected cartesian, cylindrical or spherical 1D.");
+    }
+    ErrorStatus e = WRAP_TRACEBACK(common_checks(system, bc, sp, 0));
+    if (e.return_code != SUCCESS) return e;
+    hfv_mh1d_config cfg;
+    memset(&cfg, 0, sizeof cfg);
+    cfg.num_cells_x = system->num_cells_x;
+    cfg.num_ghost_cells_side = system->num_ghost_cells_side;
+    cfg.coord_sys_flag = f;
+    cfg.gamma = system->gamma;
+    cfg.dx = system->dx_;
+    cfg.slope_limiter = ip->slope_limiter_flag_;
+    cfg.riemann_solver = ip->riemann_solver_flag_;
+    cfg.tol = ip->tol;
+    cfg.bc[0] = bc->boundary_condition_flag_x_min_;
+    cfg.bc[1] = bc->boundary_condition_flag_x_max_;
+    cfg.cfl = ip->cfl;
+    cfg.cfl_initial_shrink_factor = ip->cfl_initial_shrink_factor;
+    cfg.cfl_initial_shrink_num_steps = ip->cfl_initial_shrink_num_steps;
+    cfg.tf = prm->tf;
+    cfg.max_steps = max_steps;
+    cfg.fixed_dt = fixed_dt;
+    status->num_steps = 0;
+    status->dt = 0.0;
+    status->t = 0.0;
+    sp->store_count_ = 0;
+    double *W[3] = {system->density_, system->velocity_x_, system->pressure_};
+    double *U[3] = {system->mass_, system->momentum_x_, system->energy_};
+    hfv_run_status st;
+    const int rc = hfv_mh1d_run(&cfg, W, U, system->surface_area_x_, system->mid_points_x_, system->volume_, &st, NULL);
+    if (rc != HFV_OK) return device_error(rc);
+    fill_status(status, &st);
+    if (st.err_kind != HFV_ERR_NONE) return run_error(st.err_kind);
+    return make_success_error_status();
+}
+
+ErrorStatus muscl_hancock_2d(BoundaryConditionParam *bc, System *system, IntegratorParam *ip, StoringParam *sp,
+                             Settings *settings, SimulationParam *prm, SimulationStatus *status)
+{
+    return hfv_muscl_hancock_2d_run(bc, system, ip, sp, settings, prm, status, -1, 0.0);
+}
+
+ErrorStatus muscl_hancock_1d(BoundaryConditionParam *bc, System *system, IntegratorParam *ip, StoringParam *sp,
+                             Settings *settings, SimulationParam *prm, SimulationStatus *status)
+{
+    return hfv_muscl_hancock_1d_run(bc, system, ip, sp, settings, prm, status, -1, 0.0);
+}

@@ -1,0 +1,122 @@
+/*
+ * hfv_gpu.h -- MI355X entry points of libhydrofv_amd.so beyond the hydroFV API.
+ *
+ * The hydroFV API itself (System, launch_simulation, muscl_hancock_{1d,2d}, ...)
+ * is declared in hydro.h with the reference's layouts.  The functions here are
+ * the lower-level device interface those entry points are built on; they take
+ * plain pointers, sizes and a hipStream_t passed as void*, and return HFV_OK or
+ * a negative HFV_E* code.  Reference interfaces they implement are cited per
+ * function.
+ */
+#ifndef HYDROFV_HFV_GPU_H
+#define HYDROFV_HFV_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HFV_OK 0
+#define HFV_EINVAL (-1)   /* unsupported or inconsistent configuration */
+#define HFV_ENOMEM (-2)   /* device allocation failed */
+#define HFV_EDEVICE (-3)  /* a HIP runtime call failed */
+
+/* Device failure kinds reported by a run (see hfv_run_status.err_kind). */
+#define HFV_ERR_NONE 0
+#define HFV_ERR_DT_ZERO 1        /* "dt is zero." (integrator_muscl_hancock.c:545-549) */
+#define HFV_ERR_HLLC_SPEED 2     /* riemann_solver_hllc.c:372-375 */
+#define HFV_ERR_EXACT_BRACKET 3  /* riemann_solver_exact.c:702-707 */
+#define HFV_ERR_EXACT_BISECT 4   /* riemann_solver_exact.c:745-754 */
+
+/* Configuration of one MUSCL-Hancock run on a uniform Cartesian 2D grid.
+ * Flags use the reference's values: slope_limiter 1/2/3 (slope_limiter.h),
+ * riemann_solver 1 exact / 2 HLLC (riemann_solver.h), bc[] 1 none /
+ * 2 reflective / 3 transmissive / 4 periodic (boundary.h), in the order
+ * x_min, x_max, y_min, y_max. */
+typedef struct hfv_mh2d_config {
+    int num_cells_x, num_cells_y, num_ghost_cells_side;
+    double gamma, gravity;
+    double dx, dy;                 /* cell widths used by the CFL dt (utils.c:94) */
+    double area_x, area_y, volume; /* uniform face areas and cell volume */
+    int slope_limiter, riemann_solver;
+    double tol;
+    int bc[4];
+    double cfl, cfl_initial_shrink_factor;
+    int cfl_initial_shrink_num_steps;
+    double tf;                     /* run while t < tf */
+    long long max_steps;           /* stop after this many steps; < 0: no limit */
+    double fixed_dt;               /* > 0: constant dt instead of the CFL dt */
+    int strip_rows;                /* rows per wavefront strip; 0: default */
+} hfv_mh2d_config;
+
+typedef struct hfv_run_status {
+    double t, dt, t_prev;
+    long long num_steps;
+    int err_kind;
+    int cur;
+    long long err_cell;
+    double smax[2];
+    int finished;
+} hfv_run_status;
+
+typedef struct hfv_mh2d hfv_mh2d;
+
+/* Allocate device state for one 2D run (replaces the per-step scratch of
+ * muscl_hancock_2d, integrator_muscl_hancock.c:442-484). */
+int hfv_mh2d_create(const hfv_mh2d_config *cfg, hfv_mh2d **out);
+void hfv_mh2d_destroy(hfv_mh2d *h);
+/* Copy the caller's primitive and conserved arrays (reference layout, ghost
+ * cells included, x fastest) to the device. */
+int hfv_mh2d_upload(hfv_mh2d *h, const double *const W[4], const double *const U[4], void *stream);
+/* Reset time control, compute the first dt from W and enqueue the first step. */
+int hfv_mh2d_start(hfv_mh2d *h, void *stream);
+/* Enqueue further step launches; each one executes a step unless the run has
+ * finished (t >= tf, max_steps reached, or a failure).  use_graph replays a
+ * captured hipGraph of 48 launches. */
+int hfv_mh2d_advance(hfv_mh2d *h, long long launches, int use_graph, void *stream);
+/* Synchronise the stream and read the time-control state. */
+int hfv_mh2d_status(hfv_mh2d *h, hfv_run_status *st, void *stream);
+/* Primitive (c2p + ghost fill) and conserved state back to host arrays;
+ * either pointer array may be NULL. */
+int hfv_mh2d_download(hfv_mh2d *h, double *const W[4], double *const U[4], void *stream);
+/* Device addresses of the two conserved buffers and the primitive buffer. */
+int hfv_mh2d_fields(hfv_mh2d *h, double **U0, double **U1, double **W, long long *pitch);
+
+/* ---- 1D: muscl_hancock_1d (integrator_muscl_hancock.c:26-383) ---------- */
+typedef struct hfv_mh1d_config {
+    int num_cells_x, num_ghost_cells_side;
+    int coord_sys_flag;            /* 1 cartesian, 4 cylindrical, 5 spherical (system.h) */
+    double gamma;
+    double dx;
+    int slope_limiter, riemann_solver;
+    double tol;
+    int bc[2];
+    double cfl, cfl_initial_shrink_factor;
+    int cfl_initial_shrink_num_steps;
+    double tf;
+    long long max_steps;
+    double fixed_dt;
+} hfv_mh1d_config;
+
+/* Run the whole 1D time loop on the device in one workgroup.  W, U are the
+ * caller's host arrays (3 fields each: rho,u,p / m,mx,E) of nx+2g cells,
+ * updated in place; area has nx+2g+1 entries, mid and volume nx+2g. */
+int hfv_mh1d_run(const hfv_mh1d_config *cfg, double *const W[3], double *const U[3], const double *area,
+                 const double *mid, const double *volume, hfv_run_status *st, void *stream);
+
+/* ---- batched evaluation of the device physics (parity probes) ---------- */
+/* f[n][nf] = flux of (wl[n][nf], wr[n][nf]); nf 3 (1D) or 4 (2D); err[n]. */
+int hfv_probe_flux(int nf, int solver, double gamma, double tol, long long n, const double *wl,
+                   const double *wr, double *f, int *err);
+/* out[n] = limit_slope(a[n], b[n]) for limiter 1/2/3 (slope_limiter.c:93-115). */
+int hfv_probe_limit(int limiter, long long n, const double *a, const double *b, double *out);
+
+/* Text of the most recent HIP runtime error (for error messages). */
+const char *hfv_device_error_string(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,121 @@
+"""Problem set-ups shared by the golden-fixture generator, the oracle tests and
+the device parity tests.  Every initial state is built through the library
+API (system_init, boundary fill, primitive->conserved conversion) exactly the
+way the reference's example drivers do it, so a case can be run by the
+reference build, by libhydrofv_amd.so, or handed to the oracle."""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from hydrofv_amd import driver, workloads  # noqa: E402
+
+LIMITERS = ["minmod", "van_leer", "monotonized_central"]
+BC_SETS = {
+    "periodic": ("periodic",) * 4,
+    "transmissive": ("transmissive",) * 4,
+    "reflective": ("reflective",) * 4,
+    "mixed": ("reflective", "transmissive", "periodic", "periodic"),
+    "none": ("none", "none", "transmissive", "reflective"),
+}
+SHAPES = [(13, 7, 2), (9, 11, 3)]
+
+
+def random_state_2d(nx, ny, seed, rho=(0.5, 1.5), p=(0.5, 1.5), vel=0.3):
+    """Cell-wise random primitive state (splitmix64 stream ``seed``)."""
+    r = workloads.uniform01(seed, 4 * nx * ny).reshape(4, ny, nx)
+    return [rho[0] + (rho[1] - rho[0]) * r[0], vel * (2 * r[1] - 1), vel * (2 * r[2] - 1),
+            p[0] + (p[1] - p[0]) * r[3]]
+
+
+def setup_random_2d(lib, nx, ny, g, bcs, seed, gamma=1.4, gravity=0.0):
+    """Random interior state; every ghost cell too when a side has no
+    boundary fill ("none"), as a driver must then provide it."""
+    s = driver.Simulation(lib, "cartesian_2d", gamma, nx, ny, g, gravity=gravity)
+    s.set_boundary(*bcs)
+    if "none" in bcs:
+        s.set_primitive(random_state_2d(nx + 2 * g, ny + 2 * g, seed))
+    else:
+        s.set_primitive(random_state_2d(nx, ny, seed), interior_only=True)
+    s.apply_boundary()
+    s.primitive_to_conserved()
+    return s
+
+
+from hydrofv_amd import problems  # noqa: E402
+
+setup_kh = problems.kelvin_helmholtz
+setup_sedov = problems.sedov
+setup_sod = problems.sod
+setup_quadrant = problems.four_quadrant
+
+
+def state(sim):
+    return np.stack(sim.primitive()), np.stack(sim.conserved())
+
+
+def oracle_grid(sim, limiter, solver="riemann_solver_hllc", bcs=None, tol=1e-6, nthreads=1):
+    """Oracle state object holding a copy of a Simulation's current state."""
+    import _oracle as O
+    sysm = sim.system
+    coord = sysm.coord_sys.decode()
+    if bcs is None:
+        names = [sim.bc.boundary_condition_x_min, sim.bc.boundary_condition_x_max,
+                 sim.bc.boundary_condition_y_min, sim.bc.boundary_condition_y_max]
+        bcs = tuple(n.decode() if n else None for n in names)
+    grid = O.Grid(coord, sim.nx, sim.ny, sim.g, sysm.gamma, x=(sysm.x_min, sysm.x_max),
+                  y=(sysm.y_min, sysm.y_max), bc=bcs, limiter=limiter, solver=solver, tol=tol,
+                  gravity=sysm.gravity, nthreads=nthreads)
+    W, U = state(sim)
+    grid.W[...] = W
+    grid.U[...] = U
+    return grid
+
+
+def riemann_states(n, seed):
+    """Interface state pairs (rho, u, v, p) covering the solver branches:
+    mild jumps (PVRS guess), strong rarefactions (two-rarefaction guess and
+    vacuum generation), strong shocks (two-shock guess), vacuum on either
+    side, supersonic flow both ways, equal states and a NaN pressure."""
+    from hydrofv_amd import workloads
+    u = workloads.uniform01(seed, n * 8).reshape(n, 8)
+    wl = np.empty((n, 4))
+    wr = np.empty((n, 4))
+    wl[:, 0] = 0.1 + 2.0 * u[:, 0]
+    wr[:, 0] = 0.1 + 2.0 * u[:, 1]
+    wl[:, 1] = 2.0 * u[:, 2] - 1.0
+    wr[:, 1] = 2.0 * u[:, 3] - 1.0
+    wl[:, 2] = 2.0 * u[:, 4] - 1.0
+    wr[:, 2] = 2.0 * u[:, 5] - 1.0
+    wl[:, 3] = 0.1 + 2.0 * u[:, 6]
+    wr[:, 3] = 0.1 + 2.0 * u[:, 7]
+    cat = np.arange(n) % 10
+    # 1: strong rarefaction, 2: vacuum generation, 3: strong shock / huge pressure ratio
+    wl[cat == 1, 1] -= 2.0
+    wr[cat == 1, 1] += 2.0
+    wl[cat == 2, 1] -= 8.0
+    wr[cat == 2, 1] += 8.0
+    wl[cat == 3, 1] += 3.0
+    wr[cat == 3, 1] -= 3.0
+    wl[cat == 3, 3] *= 1e3
+    # 4/5: vacuum on the left / right; 6: both sides vacuum
+    wl[cat == 4, 0] = 0.0
+    wr[cat == 5, 0] = 0.0
+    wl[cat == 6, 0] = 0.0
+    wr[cat == 6, 0] = 0.0
+    # 7: supersonic to the right, 8: supersonic to the left
+    wl[cat == 7, 1] += 6.0
+    wr[cat == 7, 1] += 6.0
+    wl[cat == 8, 1] -= 6.0
+    wr[cat == 8, 1] -= 6.0
+    # 9: identical states (and one NaN pressure)
+    wr[cat == 9] = wl[cat == 9]
+    if n > 19:
+        wl[19, 3] = np.nan
+    return wl, wr

@@ -1,0 +1,303 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) and of the reference
+build (oracle/_ref/libhydro_ref.so).  Test infrastructure: used only by tests,
+the smoke check and the CPU-baseline leg of the benchmark, as the checker."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libhydro_ref.so")
+REF_OMP_SO = os.path.join(ROOT, "oracle", "_ref", "libhydro_ref_omp.so")
+
+dp = C.POINTER(C.c_double)
+
+COORD = {"cartesian_1d": 1, "cartesian_2d": 2, "cylindrical_1d": 4, "spherical_1d": 5}
+LIMITER = {"minmod": 1, "van_leer": 2, "monotonized_central": 3}
+SOLVER = {"riemann_solver_exact": 1, "riemann_solver_hllc": 2}
+BC = {"none": 1, "reflective": 2, "transmissive": 3, "periodic": 4}
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("coord", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("g", C.c_int),
+        ("gamma", C.c_double), ("gravity", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+        ("limiter", C.c_int), ("solver", C.c_int), ("tol", C.c_double),
+        ("bc", C.c_int * 4), ("nthreads", C.c_int),
+    ]
+
+
+class Run(C.Structure):
+    _fields_ = [
+        ("cfl", C.c_double), ("shrink_factor", C.c_double), ("shrink_steps", C.c_int),
+        ("tf", C.c_double), ("max_steps", C.c_int64), ("fixed_dt", C.c_double),
+        ("num_steps", C.c_int64), ("t", C.c_double), ("dt", C.c_double),
+        ("error", C.c_int), ("error_cell", C.c_int64),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run make -C oracle)")
+        L = C.CDLL(ORACLE_SO)
+        L.hfo_limit.restype = C.c_double
+        L.hfo_limit.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.hfo_sound_speed.restype = C.c_double
+        L.hfo_sound_speed.argtypes = [C.c_double] * 3
+        L.hfo_guess_p.restype = C.c_double
+        L.hfo_guess_p.argtypes = [C.c_double] * 10
+        for n in ("hfo_flux_hllc_1d", "hfo_flux_hllc_2d", "hfo_flux_exact_1d", "hfo_flux_exact_2d"):
+            getattr(L, n).restype = C.c_int
+            getattr(L, n).argtypes = [C.c_double, C.c_double, dp, dp, dp]
+        L.hfo_exact_1d.restype = C.c_int
+        L.hfo_exact_1d.argtypes = [C.c_double, C.c_double, dp, dp, C.c_double, dp]
+        for n in ("hfo_vacuum_1d", "hfo_vacuum_2d"):
+            getattr(L, n).restype = None
+            getattr(L, n).argtypes = [C.c_double, dp, C.c_double, dp, C.c_double, C.c_double, dp]
+        L.hfo_limit_batch.argtypes = [C.c_int, C.c_int64, dp, dp, dp]
+        L.hfo_flux_batch.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int64, dp, dp, dp,
+                                     C.POINTER(C.c_int)]
+        L.hfo_geometry.argtypes = [C.POINTER(Params)] + [C.c_double] * 4 + [dp] * 7
+        for n in ("hfo_p2c", "hfo_c2p"):
+            getattr(L, n).argtypes = [C.POINTER(Params), dp, C.POINTER(dp), C.POINTER(dp)]
+        L.hfo_bc.argtypes = [C.POINTER(Params), C.POINTER(dp)]
+        for n in ("hfo_dt_1d", "hfo_dt_2d"):
+            getattr(L, n).restype = C.c_double
+            getattr(L, n).argtypes = [C.POINTER(Params), C.POINTER(dp), C.c_double]
+        L.hfo_mh2d_run.argtypes = [C.POINTER(Params), C.POINTER(Run), dp, dp, dp, C.POINTER(dp), C.POINTER(dp)]
+        L.hfo_mh1d_run.argtypes = [C.POINTER(Params), C.POINTER(Run), dp, dp, dp, C.POINTER(dp), C.POINTER(dp)]
+        L.hfo_mh2d_step.restype = C.c_int
+        L.hfo_mh2d_step.argtypes = [C.POINTER(Params), C.c_double, dp, dp, dp, C.POINTER(dp), C.POINTER(dp),
+                                    C.POINTER(C.c_int64)]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(dp)
+
+
+def _ptrs(fields):
+    arr = (dp * len(fields))()
+    for k, a in enumerate(fields):
+        arr[k] = _ptr(a)
+    return arr
+
+
+@dataclass
+class Grid:
+    """Oracle-side state of one problem (numpy arrays, reference layout)."""
+    coord: str
+    nx: int
+    ny: int
+    g: int
+    gamma: float
+    x: tuple = (0.0, 1.0)
+    y: tuple = (0.0, 1.0)
+    bc: tuple = ("transmissive",) * 4
+    limiter: str = "minmod"
+    solver: str = "riemann_solver_hllc"
+    tol: float = 1e-6
+    gravity: float = 0.0
+    nthreads: int = 1
+    W: np.ndarray = field(default=None, repr=False)
+    U: np.ndarray = field(default=None, repr=False)
+
+    def __post_init__(self):
+        self.dim = 2 if self.coord == "cartesian_2d" else 1
+        if self.dim == 1:
+            self.ny = 1
+        self.tx = self.nx + 2 * self.g
+        self.ty = self.ny + 2 * self.g if self.dim == 2 else 1
+        nf = 4 if self.dim == 2 else 3
+        shape = (nf, self.ty, self.tx) if self.dim == 2 else (nf, self.tx)
+        if self.W is None:
+            self.W = np.zeros(shape)
+        if self.U is None:
+            self.U = np.zeros(shape)
+        self.p = Params()
+        self.p.coord = COORD[self.coord]
+        self.p.nx, self.p.ny, self.p.g = self.nx, self.ny, self.g
+        self.p.gamma, self.p.gravity = self.gamma, self.gravity
+        self.p.limiter, self.p.solver, self.p.tol = LIMITER[self.limiter], SOLVER[self.solver], self.tol
+        for k in range(4):
+            self.p.bc[k] = BC[self.bc[k]] if k < len(self.bc) and self.bc[k] else 0
+        self.p.nthreads = self.nthreads
+        self.mid_x = np.zeros(self.tx)
+        self.mid_y = np.zeros(max(self.ty, 1))
+        self.area_x = np.zeros(self.tx + 1)
+        self.area_y = np.zeros(self.ty + 1)
+        self.volume = np.zeros(self.tx * self.ty)
+        dx, dy = C.c_double(), C.c_double()
+        lib().hfo_geometry(C.byref(self.p), self.x[0], self.x[1], self.y[0], self.y[1], C.byref(dx),
+                           C.byref(dy), _ptr(self.mid_x), _ptr(self.mid_y), _ptr(self.area_x),
+                           _ptr(self.area_y), _ptr(self.volume))
+        self.p.dx, self.p.dy = dx.value, dy.value if self.dim == 2 else 0.0
+
+    def fields(self, a):
+        return [np.ascontiguousarray(a[k]) for k in range(a.shape[0])]
+
+    def _call_fields(self, fn, *arrays_groups):
+        groups = []
+        for grp in arrays_groups:
+            flat = [grp[k].reshape(-1) for k in range(grp.shape[0])]
+            for f in flat:
+                assert f.base is not None or f.flags.c_contiguous
+            groups.append(_ptrs(flat))
+        fn(*groups)
+
+    def p2c(self):
+        lib().hfo_p2c(C.byref(self.p), _ptr(self.volume), _ptrs([f.reshape(-1) for f in self.W]),
+                      _ptrs([f.reshape(-1) for f in self.U]))
+
+    def c2p(self):
+        lib().hfo_c2p(C.byref(self.p), _ptr(self.volume), _ptrs([f.reshape(-1) for f in self.U]),
+                      _ptrs([f.reshape(-1) for f in self.W]))
+
+    def apply_bc(self):
+        lib().hfo_bc(C.byref(self.p), _ptrs([f.reshape(-1) for f in self.W]))
+
+    def run(self, cfl, tf=np.inf, max_steps=-1, fixed_dt=0.0, shrink_factor=0.2, shrink_steps=10):
+        r = Run(cfl=cfl, shrink_factor=shrink_factor, shrink_steps=shrink_steps, tf=tf,
+                max_steps=max_steps, fixed_dt=fixed_dt)
+        W = _ptrs([f.reshape(-1) for f in self.W])
+        U = _ptrs([f.reshape(-1) for f in self.U])
+        if self.dim == 2:
+            lib().hfo_mh2d_run(C.byref(self.p), C.byref(r), _ptr(self.area_x), _ptr(self.area_y),
+                               _ptr(self.volume), W, U)
+        else:
+            lib().hfo_mh1d_run(C.byref(self.p), C.byref(r), _ptr(self.area_x), _ptr(self.mid_x),
+                               _ptr(self.volume), W, U)
+        return r
+
+
+def flux_batch(dim, solver, gamma, tol, wl, wr):
+    wl = np.ascontiguousarray(wl, dtype=np.float64)
+    wr = np.ascontiguousarray(wr, dtype=np.float64)
+    n = wl.shape[0]
+    f = np.zeros_like(wl)
+    err = np.zeros(n, dtype=np.int32)
+    lib().hfo_flux_batch(dim, SOLVER[solver] if isinstance(solver, str) else solver, gamma, tol, n,
+                         _ptr(wl), _ptr(wr), _ptr(f), err.ctypes.data_as(C.POINTER(C.c_int)))
+    return f, err
+
+
+def limit_batch(limiter, a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.zeros_like(a)
+    lib().hfo_limit_batch(LIMITER[limiter] if isinstance(limiter, str) else limiter, a.size, _ptr(a),
+                          _ptr(b), _ptr(out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Scalar entry points of the reference build (oracle/_ref), for pinning the
+# oracle's per-interface functions.
+_ref_scalar = None
+
+
+def ref_scalar():
+    global _ref_scalar
+    if _ref_scalar is None:
+        from hydrofv_amd import abi
+        L = C.CDLL(REF_SO)
+        E = abi.ErrorStatus
+        L.limit_slope.restype = E
+        L.limit_slope.argtypes = [dp, C.POINTER(abi.IntegratorParam), C.c_double, C.c_double]
+        L.solve_flux_hllc_1d.restype = E
+        L.solve_flux_hllc_1d.argtypes = [dp] * 3 + [C.c_double] * 8
+        L.solve_flux_hllc_2d.restype = E
+        L.solve_flux_hllc_2d.argtypes = [dp] * 4 + [C.c_double] * 10
+        L.solve_flux_exact_1d.restype = E
+        L.solve_flux_exact_1d.argtypes = [dp] * 3 + [C.c_double] * 9 + [C.c_int]
+        L.solve_exact_1d.restype = E
+        L.solve_exact_1d.argtypes = [dp] * 3 + [C.c_double] * 9 + [C.c_int]
+        L.guess_p.restype = C.c_double
+        L.guess_p.argtypes = [C.c_double] * 10
+        L.get_sound_speed.restype = C.c_double
+        L.get_sound_speed.argtypes = [C.c_double] * 3
+        L.free_traceback.argtypes = [C.POINTER(E)]
+        _ref_scalar = L
+    return _ref_scalar
+
+
+def ref_limit(limiter, a, b):
+    from hydrofv_amd import abi
+    L = ref_scalar()
+    ip = abi.IntegratorParam()
+    ip.slope_limiter_flag_ = LIMITER[limiter] if isinstance(limiter, str) else limiter
+    out = np.zeros(len(a))
+    v = C.c_double()
+    for k in range(len(a)):
+        L.limit_slope(C.byref(v), C.byref(ip), float(a[k]), float(b[k]))
+        out[k] = v.value
+    return out
+
+
+def _free(L, st):
+    if st.traceback:
+        L.free_traceback(C.byref(st))
+
+
+def ref_flux(dim, solver, gamma, tol, wl, wr):
+    """Reference flux for n interface states; err[k] = return code."""
+    L = ref_scalar()
+    n = wl.shape[0]
+    nf = 3 if dim == 1 else 4
+    f = np.full((n, nf), np.nan)
+    err = np.zeros(n, dtype=np.int32)
+    out = [C.c_double() for _ in range(nf)]
+    ptrs = [C.byref(o) for o in out]
+    for k in range(n):
+        l, r = [float(x) for x in wl[k]], [float(x) for x in wr[k]]
+        for o in out:
+            o.value = np.nan
+        if dim == 1 and solver == "riemann_solver_hllc":
+            st = L.solve_flux_hllc_1d(*ptrs, gamma, *l, *r, tol)
+        elif dim == 1:
+            st = L.solve_flux_exact_1d(*ptrs, gamma, *l, *r, tol, 0.0, 0)
+        else:
+            st = L.solve_flux_hllc_2d(*ptrs, gamma, *l, *r, tol)
+        err[k] = st.return_code
+        _free(L, st)
+        f[k] = [o.value for o in out]
+    return f, err
+
+
+def ref_exact_sample(gamma, tol, wl, wr, speeds):
+    L = ref_scalar()
+    out = [C.c_double() for _ in range(3)]
+    sol = np.zeros((len(speeds), 3))
+    for k, s in enumerate(speeds):
+        st = L.solve_exact_1d(*[C.byref(o) for o in out], gamma, *[float(x) for x in wl], *[float(x) for x in wr],
+                              tol, float(s), 0)
+        _free(L, st)
+        sol[k] = [o.value for o in out]
+    return sol
+
+
+def oracle_exact_sample(gamma, tol, wl, wr, speeds):
+    L = lib()
+    sol = np.zeros((len(speeds), 3))
+    a = np.ascontiguousarray(wl, dtype=np.float64)
+    b = np.ascontiguousarray(wr, dtype=np.float64)
+    o = np.zeros(3)
+    for k, s in enumerate(speeds):
+        L.hfo_exact_1d(gamma, tol, _ptr(a), _ptr(b), float(s), _ptr(o))
+        sol[k] = o
+    return sol
