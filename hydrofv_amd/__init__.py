@@ -32,8 +32,9 @@ def load() -> C.CDLL:
                 "or `make -C hydrofv_amd/csrc`")
         lib = C.CDLL(LIB_PATH)
         abi.bind(lib)
-        from . import gpu
+        from . import ensemble, gpu
         gpu.bind(lib)
+        ensemble.bind(lib)
         _lib = lib
     return _lib
 

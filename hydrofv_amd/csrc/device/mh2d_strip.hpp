@@ -1,0 +1,317 @@
+// mh2d_strip.hpp -- one wavefront's share of a 2D MUSCL-Hancock step.
+//
+// A wavefront (64 lanes) owns a strip of 60 interior columns plus 2 halo
+// columns on each side (lane l <-> column x0-2+l) and a run of strip_rows
+// interior rows, which it sweeps from bottom to top.  Each lane keeps a
+// 3-row window of primitive states in registers, so every cell's primitive
+// state is derived once per strip from the conserved state in HBM (W is never
+// stored between steps).  x-neighbours are exchanged with cross-lane
+// shuffles; y-neighbours are the lane's own register window.
+//
+// Per output cell and step this reads the 4 conserved fields once and writes
+// them once: 64 B of algorithmic HBM traffic; halo re-reads hit L2.
+//
+// The arithmetic per cell and per interface is that of
+// integrator_muscl_hancock.c:552-1141 (see hfv_math.hpp), with the update
+// order of the reference (x-sweep: left face then right face; y-sweep: bottom
+// face then top face, 806-814 and 1092-1100) so that U is bitwise identical.
+// Ghost primitives are formed by the boundary mapping of boundary.c:513-777
+// (x-ghosts on interior rows, y-ghosts on interior columns) applied to the
+// interior cell they copy, which yields the same bits as the reference's
+// c2p-then-fill.  The first step of a run reads the caller's primitive
+// arrays instead (the reference's first step uses the initial W as given).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hfv_ctrl.hpp"
+#include "hfv_math.hpp"
+
+namespace hfv {
+
+
+constexpr int LANES = 64;
+constexpr int HALO = 2;
+constexpr int STRIP_COLS = LANES - 2 * HALO;  // 60 output columns per wave
+
+struct Fields4 {
+    double* f[4];
+};
+
+struct Grid2D {
+    int nx, ny, g;
+    int tx, ty;
+    long long pitch;     // row pitch of every device field, in doubles
+    double gamma, gravity, tol;
+    double area_x, area_y, volume;  // uniform Cartesian geometry (system.c:400-415, 490-503)
+    int bc[4];           // x_min, x_max, y_min, y_max
+    int strip_rows;
+    int nsx, nsy;        // strips in x and y
+};
+
+// Boundary mapping of a ghost column (row) index onto the interior cell whose
+// primitive state it receives; *flip = 1 when the normal velocity is negated.
+__device__ __forceinline__ int map_index(int c, int n, int g, int bc_lo, int bc_hi, int* flip)
+{
+    *flip = 0;
+    if (c < g) {
+        if (bc_lo == BC_PERIODIC) return c + n;
+        if (bc_lo == BC_TRANSMISSIVE || bc_lo == BC_REFLECTIVE) {
+            *flip = (bc_lo == BC_REFLECTIVE);
+            return 2 * g - 1 - c;
+        }
+        return c;  // none: the ghost keeps c2p of its own conserved state
+    }
+    if (c >= g + n) {
+        if (bc_hi == BC_PERIODIC) return c - n;
+        if (bc_hi == BC_TRANSMISSIVE || bc_hi == BC_REFLECTIVE) {
+            *flip = (bc_hi == BC_REFLECTIVE);
+            return 2 * (g + n) - 1 - c;
+        }
+        return c;
+    }
+    return c;
+}
+
+// Primitive state of cell (row q, column c) as the reference's stencil sees it.
+template <bool WMODE>
+__device__ __forceinline__ void load_prim(const Grid2D& G, const Fields4& U, const Fields4& W, int q, int c,
+                                          double* w)
+{
+    q = min(max(q, 0), G.ty - 1);
+    c = min(max(c, 0), G.tx - 1);
+    if constexpr (WMODE) {
+        const long long k = (long long)q * G.pitch + c;
+        w[0] = W.f[0][k];
+        w[1] = W.f[1][k];
+        w[2] = W.f[2][k];
+        w[3] = W.f[3][k];
+    } else {
+        int fu, fv;
+        const int cs = map_index(c, G.nx, G.g, G.bc[0], G.bc[1], &fu);
+        const int qs = map_index(q, G.ny, G.g, G.bc[2], G.bc[3], &fv);
+        const long long k = (long long)qs * G.pitch + cs;
+        c2p_2d(G.gamma, G.volume, U.f[0][k], U.f[1][k], U.f[2][k], U.f[3][k], w);
+        if (fu) w[1] = -w[1];
+        if (fv) w[2] = -w[2];
+    }
+}
+
+// y-predictor of the cell whose own state is wc, with wb below and wt above.
+template <int LIM>
+__device__ __forceinline__ void ypredict(const double* wb, const double* wc, const double* wt, double c, double gamma,
+                                         double* lo, double* hi)
+{
+    double d[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++) d[f] = limit<LIM>(wc[f] - wb[f], wt[f] - wc[f]);
+    const double a = sound_speed(gamma, wc[0], wc[3]);
+    predict_y(wc, d, c, a, lo, hi);
+}
+
+// y-sweep flux between the top face of the lower cell (hi) and the bottom
+// face of the upper cell (lo): the solver sees (rho, v, u, p) and returns
+// (mass, my, mx, E) (integrator_muscl_hancock.c:1051-1067); reordered here to
+// (mass, mx, my, E).
+template <int SOLVER>
+__device__ __forceinline__ int yflux(double gamma, double tol, const double* hi, const double* lo, double* f)
+{
+    const double l[4] = {hi[0], hi[2], hi[1], hi[3]};
+    const double r[4] = {lo[0], lo[2], lo[1], lo[3]};
+    double o[4];
+    const int e = flux<SOLVER, 4>(gamma, tol, l, r, o);
+    f[0] = o[0];
+    f[1] = o[2];
+    f[2] = o[1];
+    f[3] = o[3];
+    return e;
+}
+
+__device__ __forceinline__ double shfl_from_left(double v) { return __shfl_up(v, 1, LANES); }
+__device__ __forceinline__ double shfl_from_right(double v) { return __shfl_down(v, 1, LANES); }
+
+struct StripResult {
+    double smx, smy;     // CFL maxima of the strip's updated cells
+    int err;             // ErrKind of a failed flux (0: none)
+    long long err_cell;
+};
+
+// One strip (sx, sy) of the step U -> V with time step dt.
+template <int LIM, int SOLVER, bool WMODE>
+__device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, const Fields4& U, const Fields4& V,
+                                                  const Fields4& W, int sx, int sy, int lane)
+{
+    const double cx = dt * G.area_x / G.volume;  // dt*A/V, integrator_muscl_hancock.c:636
+    const double cy = dt * G.area_y / G.volume;  // :921
+
+    const int g = G.g;
+    const int x0 = g + sx * STRIP_COLS;        // first output column of the strip
+    const int y0 = g + sy * G.strip_rows;      // first output row
+    const int y1 = min(y0 + G.strip_rows, g + G.ny);
+    const int col = x0 - HALO + lane;
+    const bool out_col = (lane >= HALO) && (lane < HALO + STRIP_COLS) && (col < g + G.nx);
+    const bool ghost_lo_col = (sx == 0) && (col == g - 1);               // x_min face flux only
+    const bool ghost_hi_col = (sx == G.nsx - 1) && (col == g + G.nx);    // x_max face flux only
+    const double gamma = G.gamma, tol = G.tol;
+
+    int err = ERR_NONE;
+    long long err_cell = -1;
+    double smx = -INFINITY, smy = -INFINITY;
+
+    // ---- prologue: rows y0-2 .. y0+1 ------------------------------------
+    double wm2[4], wm1[4], wa[4], wb[4], wc[4];
+    load_prim<WMODE>(G, U, W, y0 - 2, col, wm2);
+    load_prim<WMODE>(G, U, W, y0 - 1, col, wm1);
+    load_prim<WMODE>(G, U, W, y0, col, wa);
+    load_prim<WMODE>(G, U, W, y0 + 1, col, wb);
+    double yt[4], fyb[4];
+    {
+        double lo[4], hi_m1[4], lo0[4];
+        ypredict<LIM>(wm2, wm1, wa, cy, gamma, lo, hi_m1);   // row y0-1: top face state
+        ypredict<LIM>(wm1, wa, wb, cy, gamma, lo0, yt);      // row y0: bottom and top face states
+        const int e = yflux<SOLVER>(gamma, tol, hi_m1, lo0, fyb);
+        if (e && out_col) { err = e; err_cell = (long long)(y0 - 1) * G.tx + col; }
+    }
+    if (y0 == g && out_col) {
+        // bottom ghost row receives the y_min face flux (1092-1095)
+        const long long k = (long long)(g - 1) * G.pitch + col;
+        V.f[0][k] = U.f[0][k] - fyb[0] * dt * G.area_y;
+        V.f[1][k] = U.f[1][k] - fyb[1] * dt * G.area_y;
+        V.f[2][k] = U.f[2][k] - fyb[2] * dt * G.area_y;
+        V.f[3][k] = U.f[3][k] - fyb[3] * dt * G.area_y;
+    }
+
+    for (int r = y0; r < y1; r++) {
+        load_prim<WMODE>(G, U, W, r + 2, col, wc);
+        // y: row r+1's face states, then the flux through the top face of row r
+        double ylo[4], yhi[4], fyt[4];
+        ypredict<LIM>(wa, wb, wc, cy, gamma, ylo, yhi);
+        {
+            const int e = yflux<SOLVER>(gamma, tol, yt, ylo, fyt);
+            if (e && out_col) { err = e; err_cell = (long long)r * G.tx + col; }
+        }
+        // x: face states of row r from the lane's neighbours
+        double wl[4], wr[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            wl[f] = shfl_from_left(wa[f]);
+            wr[f] = shfl_from_right(wa[f]);
+        }
+        double xlo[4], xhi[4];
+        {
+            double d[4];
+#pragma unroll
+            for (int f = 0; f < 4; f++) d[f] = limit<LIM>(wa[f] - wl[f], wr[f] - wa[f]);
+            const double a = sound_speed(gamma, wa[0], wa[3]);
+            predict_x(wa, d, cx, a, xlo, xhi);
+        }
+        double hl[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++) hl[f] = shfl_from_left(xhi[f]);
+        double fxl[4];
+        {
+            const int e = flux<SOLVER, 4>(gamma, tol, hl, xlo, fxl);
+            if (e && (out_col || ghost_hi_col) && lane >= 1) { err = e; err_cell = (long long)r * G.tx + col - 1; }
+        }
+        double fxr[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++) fxr[f] = shfl_from_right(fxl[f]);
+
+        const long long k = (long long)r * G.pitch + col;
+        if (out_col) {
+            double u[4];
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                double x = U.f[f][k];
+                x += fxl[f] * dt * G.area_x;
+                x -= fxr[f] * dt * G.area_x;
+                x += fyb[f] * dt * G.area_y;
+                x -= fyt[f] * dt * G.area_y;
+                u[f] = x;
+            }
+            if (G.gravity != 0.0) {
+                // source_term.c:213-240 on the updated state
+                const double pre = -dt * G.gravity;
+                const double kmy = pre * u[0];
+                const double k1 = pre * u[2];
+                const double k2 = pre * (u[2] + 0.5 * kmy);
+                const double k3 = k2;
+                const double k4 = pre * (u[2] + kmy);
+                u[2] += kmy;
+                u[3] += (k1 + 2.0 * k2 + 2.0 * k3 + k4) / 6.0;
+            }
+#pragma unroll
+            for (int f = 0; f < 4; f++) V.f[f][k] = u[f];
+            // CFL maxima of the new state (utils.c:75-92)
+            double w[4];
+            c2p_2d(gamma, G.volume, u[0], u[1], u[2], u[3], w);
+            const double a = sound_speed(gamma, w[0], w[3]);
+            const double sx_ = a + fabs(w[1]);
+            const double sy_ = a + fabs(w[2]);
+            if (sx_ > smx) smx = sx_;
+            if (sy_ > smy) smy = sy_;
+        } else if (ghost_lo_col) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] - fxr[f] * dt * G.area_x;
+        } else if (ghost_hi_col) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] + fxl[f] * dt * G.area_x;
+        }
+
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            wa[f] = wb[f];
+            wb[f] = wc[f];
+            yt[f] = yhi[f];
+            fyb[f] = fyt[f];
+        }
+    }
+    if (y1 == g + G.ny && out_col) {
+        // top ghost row receives the y_max face flux (1097-1100)
+        const long long k = (long long)(g + G.ny) * G.pitch + col;
+#pragma unroll
+        for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] + fyb[f] * dt * G.area_y;
+    }
+
+    StripResult res;
+    res.smx = smx;
+    res.smy = smy;
+    res.err = err;
+    res.err_cell = err_cell;
+    return res;
+}
+
+// Fold a strip's CFL maxima and failure into the next controller slot.
+__device__ __forceinline__ void strip_finish(StripResult r, Ctrl* out, int lane)
+{
+    double smx = wave_max(r.smx);
+    double smy = wave_max(r.smy);
+    const int err = r.err;
+    const long long err_cell = r.err_cell;
+    const unsigned long long any_err = __ballot(err != ERR_NONE);
+    if (lane == 0) {
+        const unsigned long long kx = smax_key(smx), ky = smax_key(smy);
+        if (kx) atomicMax(&out->key[0], kx);
+        if (ky) atomicMax(&out->key[1], ky);
+    }
+    if (any_err) {
+        const int src = __ffsll((long long)any_err) - 1;
+        const int ek = __shfl(err, src, LANES);
+        const long long ec = __shfl(err_cell, src, LANES);
+        if (lane == 0) report_error(out, ek, ec);
+    }
+}
+
+// Consecutive workgroup ids are dealt round-robin to the 8 XCDs; remap them
+// so each XCD gets a contiguous band of tiles (neighbours share halos in L2).
+__device__ __forceinline__ int xcd_remap(int b, int ntiles)
+{
+    const int xcd = b % 8, idx = b / 8;
+    const int base = ntiles / 8, extra = ntiles % 8;
+    return xcd * base + min(xcd, extra) + idx;
+}
+
+// c2p of every cell of U into W followed by the ghost fill (mh2d.hip)
+void prim_from_u(const Grid2D& G, const Fields4& U, const Fields4& W, hipStream_t s);
+
+}  // namespace hfv

@@ -1,0 +1,149 @@
+"""Batched ensemble of independent 2D problems (hfv_ens_*) and its autotuner.
+
+The ensemble workload: `count` four-quadrant Riemann problems on square grids
+whose sides are drawn log-uniform in [64, 1024]; each instance has its own
+quadrant states, its own CFL time step and the same limiter/solver.  All
+instances advance in one launch per step (one hipGraph replay per 48 steps).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import gpu, problems, workloads
+from .gpu import MH2DConfig, RunStatus, _ptr_array, check
+
+_P = C.POINTER
+PROTOTYPES = {
+    "hfv_ens_create": (C.c_int, [C.c_int, _P(MH2DConfig), _P(C.c_void_p)]),
+    "hfv_ens_destroy": (None, [C.c_void_p]),
+    "hfv_ens_set_strip_rows": (C.c_int, [C.c_void_p, _P(C.c_int)]),
+    "hfv_ens_num_tiles": (C.c_int, [C.c_void_p]),
+    "hfv_ens_upload": (C.c_int, [C.c_void_p, C.c_int, _P(gpu.dp), _P(gpu.dp), C.c_void_p]),
+    "hfv_ens_start": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hfv_ens_advance": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_void_p]),
+    "hfv_ens_status": (C.c_int, [C.c_void_p, _P(RunStatus), C.c_void_p]),
+    "hfv_ens_download": (C.c_int, [C.c_void_p, C.c_int, _P(gpu.dp), _P(gpu.dp), C.c_void_p]),
+}
+
+
+def bind(lib):
+    for name, (restype, argtypes) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    return lib
+
+
+def build_instances(lib, count=256, seed=2, side_lo=64, side_hi=1024, steps=500, cfl=0.8,
+                    limiter=1, solver=2):
+    """Initial states (host arrays) and device configs of the ensemble."""
+    sides, states = workloads.ensemble_specs(count, seed, side_lo, side_hi)
+    cfgs, init = [], []
+    for i in range(count):
+        sim = problems.four_quadrant(lib, int(sides[i]), states[i])
+        cfgs.append(problems.device_config(sim, limiter=limiter, solver=solver, cfl=cfl, max_steps=steps))
+        init.append((sim.primitive(), sim.conserved()))
+        sim.close()
+    return sides, cfgs, init
+
+
+class Ensemble:
+    def __init__(self, lib, cfgs, init=None, stream=None):
+        self.lib = bind(lib)
+        self.count = len(cfgs)
+        arr = (MH2DConfig * self.count)(*cfgs)
+        h = C.c_void_p()
+        check(lib, lib.hfv_ens_create(self.count, arr, C.byref(h)), "hfv_ens_create")
+        self.h = h
+        self.cfgs = list(cfgs)
+        self.init = init
+        if init is not None:
+            self.upload(init, stream)
+
+    def upload(self, init, stream=None):
+        for i, (W, U) in enumerate(init):
+            W = [np.ascontiguousarray(w) for w in W]
+            U = [np.ascontiguousarray(u) for u in U]
+            check(self.lib, self.lib.hfv_ens_upload(self.h, i, _ptr_array(W), _ptr_array(U), stream), "upload")
+            # keep host arrays alive until the copies have run
+            if stream is not None and i % 16 == 15:
+                self.status(stream)
+        self.status(stream)
+
+    def set_strip_rows(self, rows):
+        a = (C.c_int * self.count)(*[int(r) for r in rows])
+        check(self.lib, self.lib.hfv_ens_set_strip_rows(self.h, a), "set_strip_rows")
+
+    @property
+    def num_tiles(self):
+        return self.lib.hfv_ens_num_tiles(self.h)
+
+    def start(self, stream=None):
+        check(self.lib, self.lib.hfv_ens_start(self.h, stream), "start")
+
+    def advance(self, launches, use_graph=True, stream=None):
+        check(self.lib, self.lib.hfv_ens_advance(self.h, launches, int(use_graph), stream), "advance")
+
+    def status(self, stream=None):
+        st = (RunStatus * self.count)()
+        check(self.lib, self.lib.hfv_ens_status(self.h, st, stream), "status")
+        return st
+
+    def download(self, i, stream=None):
+        c = self.cfgs[i]
+        shape = (c.num_cells_y + 2 * c.num_ghost_cells_side, c.num_cells_x + 2 * c.num_ghost_cells_side)
+        W = [np.empty(shape) for _ in range(4)]
+        U = [np.empty(shape) for _ in range(4)]
+        check(self.lib, self.lib.hfv_ens_download(self.h, i, _ptr_array(W), _ptr_array(U), stream), "download")
+        return W, U
+
+    def close(self):
+        if self.h:
+            self.lib.hfv_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def strip_rule(sides, kind, value):
+    """Candidate tile shapes: a fixed strip height, or a height proportional
+    to the instance's side (so small grids get short strips and more waves)."""
+    sides = np.asarray(sides)
+    if kind == "fixed":
+        return np.full(len(sides), value)
+    return np.clip(np.rint(sides / value), 8, 512).astype(int)
+
+
+def autotune(ens: Ensemble, sides, trial_launches=8, timer=None, log=None):
+    """Pick the strip heights that minimise the ensemble step time.  Each
+    candidate is timed over `trial_launches` steps from the uploaded state;
+    the state is uploaded again afterwards."""
+    candidates = [("fixed", v) for v in (16, 32, 64, 128, 256)] + [("prop", d) for d in (4, 8, 16, 32)]
+    best = None
+    results = []
+    for kind, value in candidates:
+        rows = strip_rule(sides, kind, value)
+        ens.set_strip_rows(rows)
+        ens.upload(ens.init)
+        ens.start()
+        ens.advance(2, use_graph=False)
+        ens.status()
+        t0 = time.perf_counter()
+        ens.advance(trial_launches, use_graph=False)
+        ens.status()
+        dt = (time.perf_counter() - t0) / trial_launches
+        results.append((kind, value, dt, ens.num_tiles))
+        if log:
+            log(f"autotune {kind}={value}: {dt * 1e3:.3f} ms/step, {ens.num_tiles} tiles")
+        if best is None or dt < best[2]:
+            best = (kind, value, dt, rows)
+    ens.set_strip_rows(best[3])
+    ens.upload(ens.init)
+    return best, results

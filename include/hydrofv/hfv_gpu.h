@@ -83,6 +83,25 @@ int hfv_mh2d_download(hfv_mh2d *h, double *const W[4], double *const U[4], void 
 /* Device addresses of the two conserved buffers and the primitive buffer. */
 int hfv_mh2d_fields(hfv_mh2d *h, double **U0, double **U1, double **W, long long *pitch);
 
+/* ---- batched ensemble (no reference counterpart) ---------------------- */
+/* Independent 2D problems of uneven size advanced together, one launch per
+ * step covering every strip of every instance; each instance keeps its own
+ * CFL time control and gets bitwise the result of a single-grid run.  All
+ * instances share the limiter and the Riemann solver. */
+typedef struct hfv_ens hfv_ens;
+int hfv_ens_create(int count, const hfv_mh2d_config *cfgs, hfv_ens **out);
+void hfv_ens_destroy(hfv_ens *e);
+/* Strip height per instance (the tile shape the autotuner picks). */
+int hfv_ens_set_strip_rows(hfv_ens *e, const int *rows);
+int hfv_ens_num_tiles(hfv_ens *e);
+int hfv_ens_upload(hfv_ens *e, int inst, const double *const W[4], const double *const U[4], void *stream);
+/* Reset every instance's time control, first dt from W, first step. */
+int hfv_ens_start(hfv_ens *e, void *stream);
+int hfv_ens_advance(hfv_ens *e, long long launches, int use_graph, void *stream);
+/* st[count]: state of every instance. */
+int hfv_ens_status(hfv_ens *e, hfv_run_status *st, void *stream);
+int hfv_ens_download(hfv_ens *e, int inst, double *const W[4], double *const U[4], void *stream);
+
 /* ---- 1D: muscl_hancock_1d (integrator_muscl_hancock.c:26-383) ---------- */
 typedef struct hfv_mh1d_config {
     int num_cells_x, num_ghost_cells_side;

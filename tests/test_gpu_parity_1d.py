@@ -10,7 +10,7 @@ from test_gpu_parity_2d import compare
 pytestmark = pytest.mark.gpu
 
 
-def run_1d(gpulib, setup, limiter, solver, cfl, tf=np.inf, steps=None, tol=1e-6):
+def run_1d(gpulib, setup, limiter, solver, cfl, tf=np.inf, steps=None, tol=1e-6, exact_time=True):
     sim = setup(gpulib)
     grid = K.oracle_grid(sim, limiter, solver, tol=tol)
     if steps is None:
@@ -20,7 +20,11 @@ def run_1d(gpulib, setup, limiter, solver, cfl, tf=np.inf, steps=None, tol=1e-6)
         res = sim.launch("muscl_hancock_1d", solver, limiter, cfl, tf, tol=tol, max_steps=steps)
         r = grid.run(cfl, tf=tf, max_steps=steps)
     assert r.error == 0
-    assert (res.num_steps, res.t, res.dt) == (r.num_steps, r.t, r.dt)
+    if exact_time:
+        assert (res.num_steps, res.t, res.dt) == (r.num_steps, r.t, r.dt)
+    else:
+        assert res.num_steps == r.num_steps and res.t == r.t
+        assert abs(res.dt - r.dt) <= 1e-10 * r.dt
     W, U = K.state(sim)
     return W, U, grid.W, grid.U, res
 
@@ -37,7 +41,7 @@ def test_sod_1000_hllc_minmod_bitwise(gpulib):
 def test_sod_exact_solver(gpulib, limiter):
     """Exact solver in 1D: pow() may differ from glibc in the last place."""
     W, U, Wo, Uo, res = run_1d(gpulib, lambda lib: K.setup_sod(lib, 200), limiter, "riemann_solver_exact",
-                               0.9, tf=0.2)
+                               0.9, tf=0.2, exact_time=False)
     compare("sod exact", W[:, None], U[:, None], Wo[:, None], Uo[:, None], bitwise=False, tol=1e-10)
 
 
