@@ -90,6 +90,9 @@ def main():
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after the timed steps, copy a buffer of this many MiB once on the device: a "
+                         "kernel of known traffic that scales the profiler's FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -112,8 +115,12 @@ def main():
     W = sim.primitive()
     U = sim.conserved()
     sim.close()
-    stream = torch.cuda.current_stream()
+    # A stream of our own: the library runs work handed no stream (handle 0)
+    # on a private non-blocking stream, which events on torch's default
+    # stream would not bracket.
+    stream = torch.cuda.Stream()
     sh = stream.cuda_stream
+    assert sh, "expected a non-null stream handle"
     dev.upload(W, U, stream=sh)
     del W, U
     dev.start(stream=sh)                      # step 1 (from the primitive state)
@@ -138,6 +145,15 @@ def main():
     achieved = ALGO_BYTES_PER_CELL * cells / (ms_per_step / 1e3) / 1e9
     log(f"{ms_per_step:.4f} ms/step, {value / 1e9:.2f} G cell-updates/s, {achieved:.0f} GB/s algorithmic")
     dev.close()
+
+    if args.calibrate_copy_mib:
+        src = torch.ones(args.calibrate_copy_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy_mib} MiB read + {args.calibrate_copy_mib} MiB written")
+        del src, dst
 
     cpu = None
     if not args.no_cpu_baseline:

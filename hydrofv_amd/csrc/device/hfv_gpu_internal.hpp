@@ -3,4 +3,4 @@
 
 #include <new>
 
-#include "../../../include/hydrofv/hfv_gpu.h"
+#include "hydrofv/hfv_gpu.h"

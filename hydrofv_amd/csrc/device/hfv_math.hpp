@@ -64,24 +64,46 @@ __device__ __forceinline__ double sound_speed(double gamma, double rho, double p
 __device__ __forceinline__ double coeff_A(double gamma, double rho) { return 2.0 / ((gamma + 1.0) * rho); }
 __device__ __forceinline__ double coeff_B(double gamma, double p) { return p * ((gamma - 1.0) / (gamma + 1.0)); }
 
-// riemann_solver.c:154-224: PVRS / two-rarefaction / two-shock pressure guess
-__device__ __forceinline__ double guess_p(double gamma, double rl, double ul, double pl, double al,
-                                          double rr, double ur, double pr, double ar, double tol)
+// riemann_solver.c:154-224: PVRS / two-rarefaction / two-shock pressure guess,
+// in two parts.  guess_select returns true on the two-rarefaction branch
+// (188-199) with *g = its numerator al + ar - h*(ur - ul); otherwise *g is the
+// PVRS or two-shock guess before the clamp to tol.
+__device__ __forceinline__ bool guess_select(double gamma, double rl, double ul, double pl, double al,
+                                             double rr, double ur, double pr, double ar, double* g)
 {
     const double plo = (pl > pr) ? pr : pl;
     const double phi = (pl > pr) ? pl : pr;
     const double ppv = 0.5 * (pl + pr) - 0.125 * (ur - ul) * (rl + rr) * (al + ar);
-    double guess;
     if (phi / plo <= 2.0 && (plo <= ppv && ppv <= phi)) {
-        guess = ppv;
-    } else if (ppv < plo) {
+        *g = ppv;
+        return false;
+    }
+    if (ppv < plo) {
         const double h = 0.5 * (gamma - 1.0);
-        guess = (al + ar - h * (ur - ul)) / (al / (pow(pl, h / gamma)) + ar / (pow(pr, h / gamma)));
+        *g = al + ar - h * (ur - ul);
+        return true;
+    }
+    const double gl = sqrt(coeff_A(gamma, rl) / (ppv + coeff_B(gamma, pl)));
+    const double gr = sqrt(coeff_A(gamma, rr) / (ppv + coeff_B(gamma, pr)));
+    *g = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
+    return false;
+}
+
+// Two-rarefaction guess from its numerator and pow(pl, h/gamma), pow(pr, h/gamma);
+// the caller raises the result to gamma/h (riemann_solver.c:191-199).
+__device__ __forceinline__ double trrs_base(double num, double al, double ar, double pow_l, double pow_r)
+{
+    return num / (al / (pow_l) + ar / (pow_r));
+}
+
+__device__ __forceinline__ double guess_p(double gamma, double rl, double ul, double pl, double al,
+                                          double rr, double ur, double pr, double ar, double tol)
+{
+    double guess;
+    if (guess_select(gamma, rl, ul, pl, al, rr, ur, pr, ar, &guess)) {
+        const double h = 0.5 * (gamma - 1.0);
+        guess = trrs_base(guess, al, ar, pow(pl, h / gamma), pow(pr, h / gamma));
         guess = pow(guess, gamma / h);
-    } else {
-        const double gl = sqrt(coeff_A(gamma, rl) / (ppv + coeff_B(gamma, pl)));
-        const double gr = sqrt(coeff_A(gamma, rr) / (ppv + coeff_B(gamma, pr)));
-        guess = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
     }
     if (guess < tol) guess = tol;
     return guess;
@@ -89,73 +111,69 @@ __device__ __forceinline__ double guess_p(double gamma, double rl, double ul, do
 
 // ---------------------------------------------------------------------------
 // Vacuum sampling: riemann_solver.c:239-733.  NF = 3 (rho,u,p) or 4
-// (rho,u,v,p); the pressure is the last component.  The reference hands the
-// vacuum side's own state to the sampler when one side is vacuum (435-446)
-// and measures the vacuum-generation fronts without the velocities (384-385).
-template <int NF>
-__device__ __forceinline__ void vac_right(double gamma, const double* w, double a, double speed, double* sol)
+// (rho,u,v,p); with NF = 3 the transverse velocity v is carried as 0 and
+// ignored.  The reference hands the vacuum side's own state to the sampler
+// when one side is vacuum (435-446) and measures the vacuum-generation fronts
+// without the velocities (384-385).
+struct Prim {
+    double r, u, v, p;
+};
+
+// The two rarefaction-into-vacuum samplers as one path (riemann_solver.c:
+// 239-286 for a vacuum on the right, 288-335 on the left).  sigma = +1 for
+// the right-vacuum case; the left case's expressions are the right case's
+// with a, the front offset and the (u - speed) term negated, and negation is
+// exact, so both give the bits of the reference's separate branches.
+__device__ __forceinline__ Prim vac_fan(double gamma, Prim w, double a, double speed, bool right)
 {
-    const double rho = w[0], u = w[1], p = w[NF - 1];
-    if (speed <= (u - a)) {
-        for (int k = 0; k < NF; k++) sol[k] = w[k];
-        return;
-    }
-    const double front = u + 2.0 * a / (gamma - 1.0);
-    if (speed < front) {
-        double t = (2.0 / (gamma + 1.0) + (u - speed) * (gamma - 1.0) / ((gamma + 1.0) * a));
-        t = pow(t, 2.0 / (gamma - 1.0));
-        sol[0] = rho * t;
-        sol[1] = (a + 0.5 * (gamma - 1.0) * u + speed) * 2.0 / (gamma + 1.0);
-        if (NF == 4) sol[2] = w[2];
-        sol[NF - 1] = p * pow(t, gamma);
-    } else {
-        for (int k = 0; k < NF; k++) sol[k] = 0.0;
-    }
+    const bool outside = right ? (speed <= (w.u - a)) : (speed >= (w.u + a));
+    if (outside) return w;
+    const double off = 2.0 * a / (gamma - 1.0);
+    const double front = right ? w.u + off : w.u - off;
+    const bool fan = right ? (speed < front) : (speed > front);
+    if (!fan) return Prim{0.0, 0.0, 0.0, 0.0};
+    const double x = (w.u - speed) * (gamma - 1.0) / ((gamma + 1.0) * a);
+    double t = (2.0 / (gamma + 1.0) + (right ? x : -x));
+    t = pow(t, 2.0 / (gamma - 1.0));
+    Prim o;
+    o.r = w.r * t;
+    o.u = ((right ? a : -a) + 0.5 * (gamma - 1.0) * w.u + speed) * 2.0 / (gamma + 1.0);
+    o.v = w.v;
+    o.p = w.p * pow(t, gamma);
+    return o;
 }
 
-template <int NF>
-__device__ __forceinline__ void vac_left(double gamma, const double* w, double a, double speed, double* sol)
+__device__ __forceinline__ Prim vacuum(double gamma, double rl, double ul, double vl, double pl, double al, double rr,
+                                       double ur, double vr, double pr, double ar, double speed)
 {
-    const double rho = w[0], u = w[1], p = w[NF - 1];
-    if (speed >= (u + a)) {
-        for (int k = 0; k < NF; k++) sol[k] = w[k];
-        return;
-    }
-    const double front = u - 2.0 * a / (gamma - 1.0);
-    if (speed > front) {
-        double t = (2.0 / (gamma + 1.0) - (u - speed) * (gamma - 1.0) / ((gamma + 1.0) * a));
-        t = pow(t, 2.0 / (gamma - 1.0));
-        sol[0] = rho * t;
-        sol[1] = (-a + 0.5 * (gamma - 1.0) * u + speed) * 2.0 / (gamma + 1.0);
-        if (NF == 4) sol[2] = w[2];
-        sol[NF - 1] = p * pow(t, gamma);
-    } else {
-        for (int k = 0; k < NF; k++) sol[k] = 0.0;
-    }
-}
-
-template <int NF>
-__device__ __noinline__ void vacuum(double gamma, const double* wl, double al, const double* wr, double ar,
-                                    double speed, double* sol)
-{
-    const double rl = wl[0], rr = wr[0];
+    // riemann_solver.c:377-481: which fan (if any) x/t = speed falls in
+    bool zero = false, right = true;
     if (rl <= 0.0 && rr <= 0.0) {
-        for (int k = 0; k < NF; k++) sol[k] = 0.0;
+        zero = true;
     } else if (rl <= 0.0) {
-        vac_right<NF>(gamma, wl, al, speed, sol);
+        right = true;
     } else if (rr <= 0.0) {
-        vac_left<NF>(gamma, wr, ar, speed, sol);
+        right = false;
     } else {
         const double fl = 2.0 * al / (gamma - 1.0);
         const double fr = 2.0 * ar / (gamma - 1.0);
-        if (speed <= fl) {
-            vac_right<NF>(gamma, wl, al, speed, sol);
-        } else if (fr <= speed) {
-            vac_left<NF>(gamma, wr, ar, speed, sol);
-        } else {
-            for (int k = 0; k < NF; k++) sol[k] = 0.0;
-        }
+        if (speed <= fl) right = true;
+        else if (fr <= speed) right = false;
+        else zero = true;
     }
+    if (zero) return Prim{0.0, 0.0, 0.0, 0.0};
+    const Prim w = right ? Prim{rl, ul, vl, pl} : Prim{rr, ur, vr, pr};
+    return vac_fan(gamma, w, right ? al : ar, speed, right);
+}
+
+// vacuum() on NF-component state arrays
+template <int NF>
+__device__ __forceinline__ Prim vacuum_of(double gamma, const double* wl, double al, const double* wr, double ar,
+                                          double speed)
+{
+    const double vl = (NF == 4) ? wl[2] : 0.0;
+    const double vr = (NF == 4) ? wr[2] : 0.0;
+    return vacuum(gamma, wl[0], wl[1], vl, wl[NF - 1], al, wr[0], wr[1], vr, wr[NF - 1], ar, speed);
 }
 
 // ---------------------------------------------------------------------------
@@ -168,35 +186,36 @@ __device__ __forceinline__ double hllc_q(double gamma, double pk, double ps)
     return sqrt(1.0 + (0.5 + 0.5 / gamma) * (ps / pk - 1.0));
 }
 
+// Flux of the vacuum branch (hllc.c:42-66 via riemann_solver.c:377-481).
 template <int NF>
-__device__ __forceinline__ int flux_hllc(double gamma, double tol, const double* wl, const double* wr, double* f)
+__device__ __forceinline__ void hllc_vacuum_flux(double gamma, const double* wl, double al, const double* wr,
+                                                 double ar, double* f)
+{
+    const Prim v = vacuum_of<NF>(gamma, wl, al, wr, ar, 0.0);
+    if constexpr (NF == 4) {
+        const double e = v.r * (0.5 * (v.u * v.u + v.v * v.v) + v.p / (v.r * (gamma - 1.0)));
+        f[0] = v.r * v.u;
+        f[1] = v.r * v.u * v.u + v.p;
+        f[2] = v.r * v.u * v.v;
+        f[3] = v.u * (e + v.p);
+    } else {
+        const double e = v.r * (0.5 * v.u * v.u + v.p / (v.r * (gamma - 1.0)));
+        f[0] = v.r * v.u;
+        f[1] = v.r * v.u * v.u + v.p;
+        f[2] = v.u * (e + v.p);
+    }
+}
+
+// Wave speeds and the flux of the region x/t = 0 falls in, given the
+// pressure guess ps (hllc.c:68-371).
+template <int NF>
+__device__ __forceinline__ int hllc_from_guess(double gamma, const double* wl, const double* wr, double al,
+                                               double ar, double ps, double* f)
 {
     const double rl = wl[0], ul = wl[1], pl = wl[NF - 1];
     const double rr = wr[0], ur = wr[1], pr = wr[NF - 1];
     const double vl = (NF == 4) ? wl[2] : 0.0;
     const double vr = (NF == 4) ? wr[2] : 0.0;
-    const double al = (rl > 0.0) ? sound_speed(gamma, rl, pl) : 0.0;
-    const double ar = (rr > 0.0) ? sound_speed(gamma, rr, pr) : 0.0;
-
-    if (rl <= 0.0 || rr <= 0.0 || ((al + ar) * 2.0 / (gamma + 1.0)) <= (ur - ul)) {
-        double s[NF];
-        vacuum<NF>(gamma, wl, al, wr, ar, 0.0, s);
-        if constexpr (NF == 4) {
-            const double e = s[0] * (0.5 * (s[1] * s[1] + s[2] * s[2]) + s[3] / (s[0] * (gamma - 1.0)));
-            f[0] = s[0] * s[1];
-            f[1] = s[0] * s[1] * s[1] + s[3];
-            f[2] = s[0] * s[1] * s[2];
-            f[3] = s[1] * (e + s[3]);
-        } else {
-            const double e = s[0] * (0.5 * s[1] * s[1] + s[2] / (s[0] * (gamma - 1.0)));
-            f[0] = s[0] * s[1];
-            f[1] = s[0] * s[1] * s[1] + s[2];
-            f[2] = s[1] * (e + s[2]);
-        }
-        return ERR_NONE;
-    }
-
-    const double ps = guess_p(gamma, rl, ul, pl, al, rr, ur, pr, ar, tol);
     const double sl = ul - al * hllc_q(gamma, pl, ps);
     const double sr = ur + ar * hllc_q(gamma, pr, ps);
     const double sm = (pr - pl + rl * ul * (sl - ul) - rr * ur * (sr - ur)) / (rl * (sl - ul) - rr * (sr - ur));
@@ -245,6 +264,78 @@ __device__ __forceinline__ int flux_hllc(double gamma, double tol, const double*
     return ERR_NONE;
 }
 
+__device__ __forceinline__ bool hllc_is_vacuum(double gamma, double rl, double ul, double al, double rr, double ur,
+                                               double ar)
+{
+    return rl <= 0.0 || rr <= 0.0 || ((al + ar) * 2.0 / (gamma + 1.0)) <= (ur - ul);
+}
+
+template <int NF>
+__device__ __forceinline__ int flux_hllc(double gamma, double tol, const double* wl, const double* wr, double* f)
+{
+    const double rl = wl[0], ul = wl[1], pl = wl[NF - 1];
+    const double rr = wr[0], ur = wr[1], pr = wr[NF - 1];
+    const double al = (rl > 0.0) ? sound_speed(gamma, rl, pl) : 0.0;
+    const double ar = (rr > 0.0) ? sound_speed(gamma, rr, pr) : 0.0;
+    if (hllc_is_vacuum(gamma, rl, ul, al, rr, ur, ar)) {
+        hllc_vacuum_flux<NF>(gamma, wl, al, wr, ar, f);
+        return ERR_NONE;
+    }
+    const double ps = guess_p(gamma, rl, ul, pl, al, rr, ur, pr, ar, tol);
+    return hllc_from_guess<NF>(gamma, wl, wr, al, ar, ps, f);
+}
+
+// The 2D HLLC flux in two halves around the three pow() calls of the
+// two-rarefaction guess (the only powers HLLC takes), so that a wavefront
+// can evaluate the powers of all its lanes' faces together instead of every
+// lane waiting through each face's calls (trrs_powers in mh2d_strip.hpp).
+// hllc_begin, the powers and hllc_end perform the operations of
+// flux_hllc<4> in the same order.
+enum HllcKind : int { HLLC_READY = 0, HLLC_TRRS = 1, HLLC_VACUUM = 2 };
+
+struct HllcFace {
+    double wl[4], wr[4];  // (rho, u_n, u_t, p) left and right
+    double al, ar;
+    double ps;            // pressure guess; HLLC_TRRS: the guess's numerator until the powers are taken
+    int kind;
+};
+
+__device__ __forceinline__ void hllc_begin(double gamma, double tol, const double* wl, const double* wr, HllcFace& F)
+{
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        F.wl[k] = wl[k];
+        F.wr[k] = wr[k];
+    }
+    const double rl = wl[0], ul = wl[1], pl = wl[3];
+    const double rr = wr[0], ur = wr[1], pr = wr[3];
+    F.al = (rl > 0.0) ? sound_speed(gamma, rl, pl) : 0.0;
+    F.ar = (rr > 0.0) ? sound_speed(gamma, rr, pr) : 0.0;
+    F.ps = 0.0;
+    if (hllc_is_vacuum(gamma, rl, ul, F.al, rr, ur, F.ar)) {
+        F.kind = HLLC_VACUUM;
+        return;
+    }
+    double g;
+    if (guess_select(gamma, rl, ul, pl, F.al, rr, ur, pr, F.ar, &g)) {
+        F.kind = HLLC_TRRS;
+        F.ps = g;
+        return;
+    }
+    if (g < tol) g = tol;
+    F.kind = HLLC_READY;
+    F.ps = g;
+}
+
+__device__ __forceinline__ int hllc_end(double gamma, const HllcFace& F, double* f)
+{
+    if (F.kind == HLLC_VACUUM) {
+        hllc_vacuum_flux<4>(gamma, F.wl, F.al, F.wr, F.ar, f);
+        return ERR_NONE;
+    }
+    return hllc_from_guess<4>(gamma, F.wl, F.wr, F.al, F.ar, F.ps, f);
+}
+
 // ---------------------------------------------------------------------------
 // Exact Riemann solver: riemann_solver_exact.c:332-1080 (NEWTON 100 /
 // BRACKET 100 / BISECTION 5000 iterations, lines 26-28).
@@ -265,24 +356,32 @@ __device__ __forceinline__ double wave_df(double gamma, double rho, double p, do
     return pow(ps / p, -0.5 * (gamma + 1.0) / gamma) / (p * a);
 }
 
-__device__ __forceinline__ double pressure_fn(double gamma, const double* wl, double al, const double* wr,
-                                              double ar, double ps)
+// The side states are (rho, u, p) in the exact solver (3-component order).
+struct Side {
+    double r, u, p, a;
+};
+
+__device__ __forceinline__ double pressure_fn(double gamma, Side L, Side R, double ps)
 {
-    return wave_f(gamma, wl[0], wl[2], al, ps) + wave_f(gamma, wr[0], wr[2], ar, ps) + (wr[1] - wl[1]);
+    return wave_f(gamma, L.r, L.p, L.a, ps) + wave_f(gamma, R.r, R.p, R.a, ps) + (R.u - L.u);
 }
 
-__device__ __noinline__ int solve_p_star(double gamma, const double* wl, double al, const double* wr, double ar,
-                                         double tol, double* p_star)
+struct PStar {
+    double p;
+    int rc;
+};
+
+__device__ __noinline__ PStar solve_p_star(double gamma, Side L, Side R, double tol)
 {
-    const double pg = guess_p(gamma, wl[0], wl[1], wl[2], al, wr[0], wr[1], wr[2], ar, tol);
+    const double pg = guess_p(gamma, L.r, L.u, L.p, L.a, R.r, R.u, R.p, R.a, tol);
     bool bracketed = false;
     double p_hi = pg;
-    double f_hi = pressure_fn(gamma, wl, al, wr, ar, p_hi);
+    double f_hi = pressure_fn(gamma, L, R, p_hi);
     double p_lo = 0.0, f_lo = 0.0;
     double p0 = pg;
     for (int it = 0; it < 100; it++) {
-        const double f = pressure_fn(gamma, wl, al, wr, ar, p0);
-        const double df = wave_df(gamma, wl[0], wl[2], al, p0) + wave_df(gamma, wr[0], wr[2], ar, p0);
+        const double f = pressure_fn(gamma, L, R, p0);
+        const double df = wave_df(gamma, L.r, L.p, L.a, p0) + wave_df(gamma, R.r, R.p, R.a, p0);
         if (!bracketed && f * f_hi < 0.0) {
             bracketed = true;
             f_lo = f;
@@ -291,22 +390,21 @@ __device__ __noinline__ int solve_p_star(double gamma, const double* wl, double 
         const double p1 = p0 - f / df;
         if (p1 < 0.0) break;
         if (2.0 * fabs(p1 - p0) / (p1 + p0) < tol) {
-            *p_star = p1;
-            return ERR_NONE;
+            return PStar{p1, ERR_NONE};
         }
         p0 = p1;
     }
     if (!bracketed) {
         p_lo = 0.0;
-        f_lo = pressure_fn(gamma, wl, al, wr, ar, p_lo);
+        f_lo = pressure_fn(gamma, L, R, p_lo);
         if (f_lo * f_hi >= 0.0) {
             p_hi *= 10.0;
             const double dp = (p_hi - p_lo) / 100;
             for (int k = 0; k < 100; k++) {
                 const double a = p_lo + k * dp;
                 const double b = p_lo + (k + 1) * dp;
-                const double fa = pressure_fn(gamma, wl, al, wr, ar, a);
-                const double fb = pressure_fn(gamma, wl, al, wr, ar, b);
+                const double fa = pressure_fn(gamma, L, R, a);
+                const double fb = pressure_fn(gamma, L, R, b);
                 if (fa * fb < 0.0) {
                     p_lo = a;
                     f_lo = fa;
@@ -315,17 +413,16 @@ __device__ __noinline__ int solve_p_star(double gamma, const double* wl, double 
                     break;
                 }
             }
-            if (f_lo * f_hi >= 0.0) return ERR_EXACT_BRACKET;
+            if (f_lo * f_hi >= 0.0) return PStar{0.0, ERR_EXACT_BRACKET};
         }
     }
     int count = 0;
     for (;;) {
         const double pm = 0.5 * (p_lo + p_hi);
         if (p_hi - p_lo < tol) {
-            *p_star = pm;
-            return ERR_NONE;
+            return PStar{pm, ERR_NONE};
         }
-        const double fm = pressure_fn(gamma, wl, al, wr, ar, pm);
+        const double fm = pressure_fn(gamma, L, R, pm);
         if (fm * f_lo < 0.0) {
             p_hi = pm;
             f_hi = fm;
@@ -334,134 +431,139 @@ __device__ __noinline__ int solve_p_star(double gamma, const double* wl, double 
             f_lo = fm;
         }
         count++;
-        if (count > 5000) return ERR_EXACT_BISECT;
+        if (count > 5000) return PStar{0.0, ERR_EXACT_BISECT};
     }
 }
 
-// exact.c:765-974 (left), 817-1080 (right; note its own operand orders)
-__device__ __forceinline__ void sample_left(double gamma, const double* w, double a, double us, double ps,
-                                            double speed, double* sol)
+// exact.c:765-974 (left), 817-1080 (right; note its own operand orders).
+// Returns (rho, u, p) in Prim's r, u, p.
+__device__ __forceinline__ Prim sample_left(double gamma, Side W, double us, double ps, double speed)
 {
-    const double rho = w[0], u = w[1], p = w[2];
+    const double rho = W.r, u = W.u, p = W.p, a = W.a;
+    Prim o{rho, u, 0.0, p};
     if (ps > p) {
         const double ratio = ps / p;
         const double s = u - a * sqrt((gamma + 1.0) / (2.0 * gamma) * ratio + (gamma - 1.0) / (2.0 * gamma));
-        if (speed < s) {
-            sol[0] = rho; sol[1] = u; sol[2] = p;
-        } else {
+        if (!(speed < s)) {
             const double gg = (gamma - 1.0) / (gamma + 1.0);
-            sol[0] = (rho * (ratio + gg) / (gg * ratio + 1.0));
-            sol[1] = us;
-            sol[2] = ps;
+            o.r = (rho * (ratio + gg) / (gg * ratio + 1.0));
+            o.u = us;
+            o.p = ps;
         }
-        return;
+        return o;
     }
-    if (speed < u - a) {
-        sol[0] = rho; sol[1] = u; sol[2] = p;
-        return;
-    }
+    if (speed < u - a) return o;
     const double a_star = a * pow(ps / p, (gamma - 1.0) / (2.0 * gamma));
     if (speed < us - a_star) {
         const double c2 = 2.0 / (gamma + 1.0);
         const double common = pow((c2 + (u - speed) * ((gamma - 1.0) / (a * (gamma + 1.0)))), 2.0 / (gamma - 1.0));
-        sol[0] = rho * common;
-        sol[1] = c2 * (a + 0.5 * u * (gamma - 1.0) + speed);
-        sol[2] = p * pow(common, gamma);
+        o.r = rho * common;
+        o.u = c2 * (a + 0.5 * u * (gamma - 1.0) + speed);
+        o.p = p * pow(common, gamma);
     } else {
-        sol[0] = rho * pow(ps / p, 1.0 / gamma);
-        sol[1] = us;
-        sol[2] = ps;
+        o.r = rho * pow(ps / p, 1.0 / gamma);
+        o.u = us;
+        o.p = ps;
     }
+    return o;
 }
 
-__device__ __forceinline__ void sample_right(double gamma, const double* w, double a, double us, double ps,
-                                             double speed, double* sol)
+__device__ __forceinline__ Prim sample_right(double gamma, Side W, double us, double ps, double speed)
 {
-    const double rho = w[0], u = w[1], p = w[2];
+    const double rho = W.r, u = W.u, p = W.p, a = W.a;
+    Prim o{rho, u, 0.0, p};
     if (ps > p) {
         const double ratio = ps / p;
         const double s = u + a * sqrt(ratio * (gamma + 1.0) / (2.0 * gamma) + (gamma - 1.0) / (2.0 * gamma));
-        if (speed > s) {
-            sol[0] = rho; sol[1] = u; sol[2] = p;
-        } else {
+        if (!(speed > s)) {
             const double gg = (gamma - 1.0) / (gamma + 1.0);
-            sol[0] = (rho * (ratio + gg) / (gg * ratio + 1.0));
-            sol[1] = us;
-            sol[2] = ps;
+            o.r = (rho * (ratio + gg) / (gg * ratio + 1.0));
+            o.u = us;
+            o.p = ps;
         }
-        return;
+        return o;
     }
-    if (speed > u + a) {
-        sol[0] = rho; sol[1] = u; sol[2] = p;
-        return;
-    }
+    if (speed > u + a) return o;
     const double a_star = a * pow(ps / p, (gamma - 1.0) / (2.0 * gamma));
     if (speed > us + a_star) {
         const double c2 = 2.0 / (gamma + 1.0);
         const double common = pow((c2 - (u - speed) * (gamma - 1.0) / (a * (gamma + 1.0))), 2.0 / (gamma - 1.0));
-        sol[0] = rho * common;
-        sol[1] = c2 * (-a + 0.5 * u * (gamma - 1.0) + speed);
-        sol[2] = p * pow(common, gamma);
+        o.r = rho * common;
+        o.u = c2 * (-a + 0.5 * u * (gamma - 1.0) + speed);
+        o.p = p * pow(common, gamma);
     } else {
-        sol[0] = rho * pow(ps / p, 1.0 / gamma);
-        sol[1] = us;
-        sol[2] = ps;
+        o.r = rho * pow(ps / p, 1.0 / gamma);
+        o.u = us;
+        o.p = ps;
     }
+    return o;
 }
 
-// exact.c:332-469 at a given x/t; *us_out = u* (NaN on the vacuum branch)
-__device__ __noinline__ int exact_sample(double gamma, double tol, const double* wl, const double* wr,
-                                         double speed, double* sol, double* us_out)
+// exact.c:332-469 at a given x/t.  w = (rho, u, v, p) of the sampled state:
+// v is the transverse velocity carried by the contact (vl where x/t < u*,
+// else vr) or by the vacuum sampler; us = u* (NaN on the vacuum branch).
+// With vl = vr = 0 this is the 1D sampler.
+struct Sample {
+    Prim w;
+    double us;
+    int rc;
+};
+
+__device__ __noinline__ Sample exact_sample(double gamma, double tol, double rl, double ul, double vl, double pl,
+                                            double rr, double ur, double vr, double pr, double speed)
 {
-    const double al = (wl[0] > 0.0) ? sound_speed(gamma, wl[0], wl[2]) : 0.0;
-    const double ar = (wr[0] > 0.0) ? sound_speed(gamma, wr[0], wr[2]) : 0.0;
-    *us_out = __builtin_nan("");
-    if (wl[0] <= 0.0 || wr[0] <= 0.0 || ((al + ar) * 2.0 / (gamma + 1.0)) <= (wr[1] - wl[1])) {
-        vacuum<3>(gamma, wl, al, wr, ar, speed, sol);
-        return ERR_NONE;
+    const double al = (rl > 0.0) ? sound_speed(gamma, rl, pl) : 0.0;
+    const double ar = (rr > 0.0) ? sound_speed(gamma, rr, pr) : 0.0;
+    Sample S;
+    S.us = __builtin_nan("");
+    S.rc = ERR_NONE;
+    if (rl <= 0.0 || rr <= 0.0 || ((al + ar) * 2.0 / (gamma + 1.0)) <= (ur - ul)) {
+        S.w = vacuum(gamma, rl, ul, vl, pl, al, rr, ur, vr, pr, ar, speed);
+        return S;
     }
-    double ps;
-    const int rc = solve_p_star(gamma, wl, al, wr, ar, tol, &ps);
-    if (rc != ERR_NONE) return rc;
-    const double us = 0.5 * (wl[1] + wr[1]) + 0.5 * (wave_f(gamma, wr[0], wr[2], ar, ps) - wave_f(gamma, wl[0], wl[2], al, ps));
-    *us_out = us;
-    if (speed < us) sample_left(gamma, wl, al, us, ps, speed, sol);
-    else sample_right(gamma, wr, ar, us, ps, speed, sol);
-    return ERR_NONE;
+    const Side L{rl, ul, pl, al}, R{rr, ur, pr, ar};
+    const PStar P = solve_p_star(gamma, L, R, tol);
+    if (P.rc != ERR_NONE) {
+        const double nan = __builtin_nan("");
+        S.w = Prim{nan, nan, nan, nan};
+        S.rc = P.rc;
+        return S;
+    }
+    const double ps = P.p;
+    const double us = 0.5 * (ul + ur) + 0.5 * (wave_f(gamma, rr, pr, ar, ps) - wave_f(gamma, rl, pl, al, ps));
+    S.us = us;
+    if (speed < us) {
+        S.w = sample_left(gamma, L, us, ps, speed);
+        S.w.v = vl;
+    } else {
+        S.w = sample_right(gamma, R, us, ps, speed);
+        S.w.v = vr;
+    }
+    return S;
 }
 
 // Exact Godunov flux at x/t = 0.  NF = 3: exact.c:471-510.  NF = 4: the
 // normal components are computed exactly as in 1D; the transverse velocity is
 // advected by the contact (v_L when 0 < u*, the "speed < u_star" side of
-// exact.c:431, else v_R); on the vacuum branch the 2D vacuum sampler supplies
-// it.  At v = 0 the 2D flux equals the 1D flux bitwise.
+// exact.c:431, else v_R); on the vacuum branch the vacuum sampler carries it.
+// At v = 0 the 2D flux equals the 1D flux bitwise.
 template <int NF>
 __device__ __forceinline__ int flux_exact(double gamma, double tol, const double* wl, const double* wr, double* f)
 {
-    const double l3[3] = {wl[0], wl[1], wl[NF - 1]};
-    const double r3[3] = {wr[0], wr[1], wr[NF - 1]};
-    double s[3] = {__builtin_nan(""), __builtin_nan(""), __builtin_nan("")};
-    double us;
-    const int rc = exact_sample(gamma, tol, l3, r3, 0.0, s, &us);
-    f[0] = s[0] * s[1];
-    f[1] = s[0] * s[1] * s[1] + s[2];
+    const double vl = (NF == 4) ? wl[2] : 0.0;
+    const double vr = (NF == 4) ? wr[2] : 0.0;
+    const Sample S = exact_sample(gamma, tol, wl[0], wl[1], vl, wl[NF - 1], wr[0], wr[1], vr, wr[NF - 1], 0.0);
+    const double r = S.w.r, u = S.w.u, p = S.w.p;
+    f[0] = r * u;
+    f[1] = r * u * u + p;
     if constexpr (NF == 4) {
-        double v;
-        if (isnan(us)) {
-            const double al = (wl[0] > 0.0) ? sound_speed(gamma, wl[0], wl[3]) : 0.0;
-            const double ar = (wr[0] > 0.0) ? sound_speed(gamma, wr[0], wr[3]) : 0.0;
-            double s4[4];
-            vacuum<4>(gamma, wl, al, wr, ar, 0.0, s4);
-            v = s4[2];
-        } else {
-            v = (0.0 < us) ? wl[2] : wr[2];
-        }
+        const double v = S.w.v;
         f[2] = f[0] * v;
-        f[3] = (s[2] * (gamma / (gamma - 1.0)) + (0.5 * s[0] * s[1] * s[1] + 0.5 * s[0] * v * v)) * s[1];
+        f[3] = (p * (gamma / (gamma - 1.0)) + (0.5 * r * u * u + 0.5 * r * v * v)) * u;
     } else {
-        f[2] = (s[2] * (gamma / (gamma - 1.0)) + 0.5 * s[0] * s[1] * s[1]) * s[1];
+        f[2] = (p * (gamma / (gamma - 1.0)) + 0.5 * r * u * u) * u;
     }
-    return rc;
+    return S.rc;
 }
 
 template <int SOLVER, int NF>

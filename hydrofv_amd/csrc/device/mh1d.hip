@@ -13,6 +13,7 @@
 #include "hfv_ctrl.hpp"
 #include "hfv_math.hpp"
 #include "hfv_gpu_internal.hpp"
+#include "mh2d_strip.hpp"
 
 namespace hfv {
 
@@ -226,6 +227,41 @@ __global__ void k_probe_flux4(int solver, double gamma, double tol, long long n,
     err[i] = e;
 }
 
+// The 2D kernels' HLLC form: hllc_begin, the wavefront's two-rarefaction
+// powers, hllc_end.  Lane l of a one-wavefront block takes state pairs i and
+// n + i (when they exist), so every lane holds two faces like a strip row.
+// Lanes past the end work on pair 0 and store nothing.
+__global__ void __launch_bounds__(LANES) k_probe_flux4_wave(double gamma, double tol, long long n, long long half,
+                                                            const double* wl, const double* wr, double* f, int* err)
+{
+    __shared__ double q[6 * LANES];
+    const int lane = threadIdx.x;
+    const long long i = (long long)blockIdx.x * LANES + lane;
+    const long long ia = (i < half) ? i : 0, ib = (half + i < n) ? half + i : 0;
+    double la[4], ra[4], lb[4], rb[4];
+    for (int k = 0; k < 4; k++) {
+        la[k] = wl[4 * ia + k];
+        ra[k] = wr[4 * ia + k];
+        lb[k] = wl[4 * ib + k];
+        rb[k] = wr[4 * ib + k];
+    }
+    HllcFace A, B;
+    hllc_begin(gamma, tol, la, ra, A);
+    hllc_begin(gamma, tol, lb, rb, B);
+    trrs_powers(gamma, tol, A, B, q, lane);
+    double oa[4], ob[4];
+    const int ea = hllc_end(gamma, A, oa);
+    const int eb = hllc_end(gamma, B, ob);
+    if (i < half) {
+        for (int k = 0; k < 4; k++) f[4 * ia + k] = oa[k];
+        err[ia] = ea;
+    }
+    if (half + i < n) {
+        for (int k = 0; k < 4; k++) f[4 * ib + k] = ob[k];
+        err[ib] = eb;
+    }
+}
+
 __global__ void k_probe_limit(int lim, long long n, const double* a, const double* b, double* out)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -361,8 +397,15 @@ int hfv_probe_flux(int nf, int solver, double gamma, double tol, long long n, co
         hipMemcpy(dr.p, wr, m * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         return HFV_EDEVICE;
     const dim3 grid((unsigned)((n + 255) / 256)), block(256);
-    if (nf == 3) hipLaunchKernelGGL(k_probe_flux3, grid, block, 0, 0, solver, gamma, tol, n, dl.p, dr.p, df.p, de.p);
-    else hipLaunchKernelGGL(k_probe_flux4, grid, block, 0, 0, solver, gamma, tol, n, dl.p, dr.p, df.p, de.p);
+    if (nf == 3) {
+        hipLaunchKernelGGL(k_probe_flux3, grid, block, 0, 0, solver, gamma, tol, n, dl.p, dr.p, df.p, de.p);
+    } else if (solver == HLLC) {
+        const long long half = (n + 1) / 2;
+        const dim3 wgrid((unsigned)((half + LANES - 1) / LANES)), wblock(LANES);
+        hipLaunchKernelGGL(k_probe_flux4_wave, wgrid, wblock, 0, 0, gamma, tol, n, half, dl.p, dr.p, df.p, de.p);
+    } else {
+        hipLaunchKernelGGL(k_probe_flux4, grid, block, 0, 0, solver, gamma, tol, n, dl.p, dr.p, df.p, de.p);
+    }
     if (hipGetLastError() != hipSuccess) return HFV_EDEVICE;
     if (hipMemcpy(f, df.p, m * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(err, de.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)

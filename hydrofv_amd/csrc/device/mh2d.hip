@@ -188,8 +188,8 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
     G.area_y = cfg->area_y;
     G.volume = cfg->volume;
     for (int s = 0; s < 4; s++) G.bc[s] = cfg->bc[s];
-    G.strip_rows = cfg->strip_rows > 0 ? cfg->strip_rows : 64;
     G.nsx = (G.nx + STRIP_COLS - 1) / STRIP_COLS;
+    G.strip_rows = cfg->strip_rows > 0 ? cfg->strip_rows : default_strip_rows(G.nx, G.ny);
     G.nsy = (G.ny + G.strip_rows - 1) / G.strip_rows;
     RunParams& rp = h->rp;
     rp.cfl = cfg->cfl;

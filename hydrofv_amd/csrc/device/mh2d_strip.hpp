@@ -127,6 +127,57 @@ __device__ __forceinline__ int yflux(double gamma, double tol, const double* hi,
     return e;
 }
 
+// The powers of the two-rarefaction pressure guesses pending on two faces per
+// lane (riemann_solver.c:191-199), evaluated 64 at a time over the
+// wavefront: the lanes' pending bases are packed into LDS, each lane raises
+// one packed value per pass, and the results are read back.  A row of a
+// strip has about 30 such faces in 128, so this takes two or three pow()
+// sequences per row instead of six.  q: 384 doubles of LDS; the workgroup is
+// this one wavefront, and every lane must call this.
+__device__ __forceinline__ void trrs_powers(double gamma, double tol, HllcFace& A, HllcFace& B, double* q, int lane)
+{
+    const bool ta = A.kind == HLLC_TRRS, tb = B.kind == HLLC_TRRS;
+    const unsigned long long ma = __ballot(ta), mb = __ballot(tb);
+    if ((ma | mb) == 0ull) return;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int na = __popcll(ma), n = na + __popcll(mb);
+    const int ia = __popcll(ma & below), ib = na + __popcll(mb & below);
+    const double h = 0.5 * (gamma - 1.0);
+
+    // stage 1: pow(pl, h/gamma) and pow(pr, h/gamma) at q[2i], q[2i+1]
+    if (ta) { q[2 * ia] = A.wl[3]; q[2 * ia + 1] = A.wr[3]; }
+    if (tb) { q[2 * ib] = B.wl[3]; q[2 * ib + 1] = B.wr[3]; }
+    __syncthreads();
+    const double e1 = h / gamma;
+    for (int b = 0; b < 2 * n; b += LANES) {
+        const int i = b + lane;
+        if (i < 2 * n) q[i] = pow(q[i], e1);
+    }
+    __syncthreads();
+    // stage 2: pow(base, gamma/h) at q[256 + i]
+    double* q2 = q + 4 * LANES;
+    if (ta) q2[ia] = trrs_base(A.ps, A.al, A.ar, q[2 * ia], q[2 * ia + 1]);
+    if (tb) q2[ib] = trrs_base(B.ps, B.al, B.ar, q[2 * ib], q[2 * ib + 1]);
+    __syncthreads();
+    const double e2 = gamma / h;
+    for (int b = 0; b < n; b += LANES) {
+        const int i = b + lane;
+        if (i < n) q2[i] = pow(q2[i], e2);
+    }
+    __syncthreads();
+    if (ta) {
+        A.ps = q2[ia];
+        if (A.ps < tol) A.ps = tol;
+        A.kind = HLLC_READY;
+    }
+    if (tb) {
+        B.ps = q2[ib];
+        if (B.ps < tol) B.ps = tol;
+        B.kind = HLLC_READY;
+    }
+    __syncthreads();
+}
+
 __device__ __forceinline__ double shfl_from_left(double v) { return __shfl_up(v, 1, LANES); }
 __device__ __forceinline__ double shfl_from_right(double v) { return __shfl_down(v, 1, LANES); }
 
@@ -157,6 +208,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     int err = ERR_NONE;
     long long err_cell = -1;
     double smx = -INFINITY, smy = -INFINITY;
+    __shared__ double trrs_q[6 * LANES];
 
     // ---- prologue: rows y0-2 .. y0+1 ------------------------------------
     double wm2[4], wm1[4], wa[4], wb[4], wc[4];
@@ -183,13 +235,10 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
 
     for (int r = y0; r < y1; r++) {
         load_prim<WMODE>(G, U, W, r + 2, col, wc);
-        // y: row r+1's face states, then the flux through the top face of row r
-        double ylo[4], yhi[4], fyt[4];
+        // y: row r+1's face states (the flux through the top face of row r
+        // follows below)
+        double ylo[4], yhi[4];
         ypredict<LIM>(wa, wb, wc, cy, gamma, ylo, yhi);
-        {
-            const int e = yflux<SOLVER>(gamma, tol, yt, ylo, fyt);
-            if (e && out_col) { err = e; err_cell = (long long)r * G.tx + col; }
-        }
         // x: face states of row r from the lane's neighbours
         double wl[4], wr[4];
 #pragma unroll
@@ -208,11 +257,32 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         double hl[4];
 #pragma unroll
         for (int f = 0; f < 4; f++) hl[f] = shfl_from_left(xhi[f]);
-        double fxl[4];
-        {
-            const int e = flux<SOLVER, 4>(gamma, tol, hl, xlo, fxl);
-            if (e && (out_col || ghost_hi_col) && lane >= 1) { err = e; err_cell = (long long)r * G.tx + col - 1; }
+        double fyt[4], fxl[4];
+        int ey, ex;
+        if constexpr (SOLVER == HLLC) {
+            // both faces' fluxes, with the wavefront's two-rarefaction
+            // powers taken together
+            HllcFace Fy, Fx;
+            {
+                const double l[4] = {yt[0], yt[2], yt[1], yt[3]};
+                const double rr[4] = {ylo[0], ylo[2], ylo[1], ylo[3]};
+                hllc_begin(gamma, tol, l, rr, Fy);
+            }
+            hllc_begin(gamma, tol, hl, xlo, Fx);
+            trrs_powers(gamma, tol, Fy, Fx, trrs_q, lane);
+            double o[4];
+            ey = hllc_end(gamma, Fy, o);
+            fyt[0] = o[0];
+            fyt[1] = o[2];
+            fyt[2] = o[1];
+            fyt[3] = o[3];
+            ex = hllc_end(gamma, Fx, fxl);
+        } else {
+            ey = yflux<SOLVER>(gamma, tol, yt, ylo, fyt);
+            ex = flux<SOLVER, 4>(gamma, tol, hl, xlo, fxl);
         }
+        if (ey && out_col) { err = ey; err_cell = (long long)r * G.tx + col; }
+        if (ex && (out_col || ghost_hi_col) && lane >= 1) { err = ex; err_cell = (long long)r * G.tx + col - 1; }
         double fxr[4];
 #pragma unroll
         for (int f = 0; f < 4; f++) fxr[f] = shfl_from_right(fxl[f]);
@@ -300,6 +370,18 @@ __device__ __forceinline__ void strip_finish(StripResult r, Ctrl* out, int lane)
         const long long ec = __shfl(err_cell, src, LANES);
         if (lane == 0) report_error(out, ek, ec);
     }
+}
+
+// Strip height when the caller leaves it to the library: 32 rows (each
+// strip re-derives 4 halo rows, about 12% extra work at 32), halved down to
+// 8 while the grid has fewer than 4096 strips, so that a small grid still
+// puts several wavefronts on each of the chip's 1024 SIMDs.
+inline int default_strip_rows(int nx, int ny)
+{
+    const long long nsx = (nx + STRIP_COLS - 1) / STRIP_COLS;
+    int rows = 32;
+    while (rows > 8 && nsx * ((ny + rows - 1) / rows) < 4096) rows /= 2;
+    return rows;
 }
 
 // Consecutive workgroup ids are dealt round-robin to the 8 XCDs; remap them

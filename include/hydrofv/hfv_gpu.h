@@ -47,7 +47,7 @@ typedef struct hfv_mh2d_config {
     double tf;                     /* run while t < tf */
     long long max_steps;           /* stop after this many steps; < 0: no limit */
     double fixed_dt;               /* > 0: constant dt instead of the CFL dt */
-    int strip_rows;                /* rows per wavefront strip; 0: default */
+    int strip_rows;                /* rows per wavefront strip; 0: 32, fewer for small grids */
 } hfv_mh2d_config;
 
 typedef struct hfv_run_status {

@@ -119,3 +119,14 @@ def riemann_states(n, seed):
     if n > 19:
         wl[19, 3] = np.nan
     return wl, wr
+
+
+def rel_err(a, b):
+    """Largest absolute difference over the finite entries of b, relative to
+    the largest magnitude among them."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    fin = np.isfinite(b)
+    scale = np.maximum(np.abs(b[fin]).max() if fin.any() else 1.0, 1e-300)
+    d = np.abs(a[fin] - b[fin]).max() if fin.any() else 0.0
+    return d / scale

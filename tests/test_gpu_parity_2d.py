@@ -74,15 +74,6 @@ def test_probe_fluxes(gpulib, nf, solver):
         assert pow_free.sum() > 1000
 
 
-def rel_err(a, b):
-    a = np.asarray(a)
-    b = np.asarray(b)
-    fin = np.isfinite(b)
-    scale = np.maximum(np.abs(b[fin]).max() if fin.any() else 1.0, 1e-300)
-    d = np.abs(a[fin] - b[fin]).max() if fin.any() else 0.0
-    return d / scale
-
-
 def compare(tag, Wg, Ug, Wo, Uo, bitwise=False, tol=TOL, mask=None):
     """Every field within `tol` (relative to the field's largest magnitude);
     bitwise when requested.  Returns whether all fields are bitwise equal."""
@@ -94,7 +85,7 @@ def compare(tag, Wg, Ug, Wo, Uo, bitwise=False, tol=TOL, mask=None):
                 a, b = a[mask], b[mask]
             same_nan = np.array_equal(np.isnan(a), np.isnan(b))
             assert same_nan, f"{tag}: NaN pattern differs in {name}[{k}]"
-            err = rel_err(a, b)
+            err = K.rel_err(a, b)
             assert err <= tol, f"{tag}: {name}[{k}] relative error {err:.3e} > {tol}"
             eq = np.array_equal(a, b, equal_nan=True)
             same = same and eq
