@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=96,
+                    help="steps of the 256-instance batched ensemble and of its sequential baseline (0: skip)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB once on the device: a "
                          "kernel of known traffic that scales the profiler's FETCH_SIZE/WRITE_SIZE counters")
@@ -172,6 +174,15 @@ def main():
         except Exception as e:  # the GPU figure stands without it
             log(f"cpu baseline failed: {e!r}")
 
+    ens = None
+    if args.ensemble_steps:
+        from hydrofv_amd import ensemble
+        try:
+            ens = ensemble.benchmark(lib, count=256, steps=args.ensemble_steps, log=log)
+            log(f"ensemble: {ens}")
+        except Exception as e:  # the main figure stands without it
+            log(f"ensemble benchmark failed: {e!r}")
+
     traffic = measured_traffic()
     line = {
         "metric": "cell-updates/s FP64 MUSCL-HLLC 4096²; % HBM roofline; batched & adjoint ratio",
@@ -192,6 +203,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     print(json.dumps(line), flush=True)
 

@@ -121,23 +121,23 @@ def strip_rule(sides, kind, value):
     return np.clip(np.rint(sides / value), 8, 512).astype(int)
 
 
-def autotune(ens: Ensemble, sides, trial_launches=8, timer=None, log=None):
+def autotune(ens: Ensemble, sides, trial_launches=8, log=None, stream=None):
     """Pick the strip heights that minimise the ensemble step time.  Each
-    candidate is timed over `trial_launches` steps from the uploaded state;
-    the state is uploaded again afterwards."""
-    candidates = [("fixed", v) for v in (16, 32, 64, 128, 256)] + [("prop", d) for d in (4, 8, 16, 32)]
+    candidate rule is timed over `trial_launches` steps continuing from the
+    current device state (the step cost hardly depends on the state); the
+    batch's initial state is uploaded once afterwards."""
+    candidates = [("fixed", v) for v in (16, 32, 64)] + [("prop", d) for d in (4, 8, 16, 32)]
     best = None
     results = []
+    ens.start(stream)
     for kind, value in candidates:
         rows = strip_rule(sides, kind, value)
         ens.set_strip_rows(rows)
-        ens.upload(ens.init)
-        ens.start()
-        ens.advance(2, use_graph=False)
-        ens.status()
+        ens.advance(2, use_graph=False, stream=stream)
+        ens.status(stream)
         t0 = time.perf_counter()
-        ens.advance(trial_launches, use_graph=False)
-        ens.status()
+        ens.advance(trial_launches, use_graph=False, stream=stream)
+        ens.status(stream)
         dt = (time.perf_counter() - t0) / trial_launches
         results.append((kind, value, dt, ens.num_tiles))
         if log:
@@ -145,5 +145,62 @@ def autotune(ens: Ensemble, sides, trial_launches=8, timer=None, log=None):
         if best is None or dt < best[2]:
             best = (kind, value, dt, rows)
     ens.set_strip_rows(best[3])
-    ens.upload(ens.init)
+    if ens.init is not None:
+        ens.upload(ens.init, stream)
     return best, results
+
+
+def benchmark(lib, count=256, seed=2, steps=96, log=None):
+    assert steps % 48 == 0, "whole graph replays only"
+    """Aggregate cell-updates/s of the batched engine (autotuned strips, one
+    launch per step, hipGraph replays) against the same instances run one
+    after another on the single-grid engine (its own graph replays and
+    default strips), both timed over `steps` steps from the initial state
+    with the data resident on the device."""
+    import torch
+
+    sides, cfgs, init = build_instances(lib, count=count, seed=seed, steps=-1)
+    cells = int(np.sum(sides.astype(np.int64) ** 2))
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    ens = Ensemble(lib, cfgs, init, stream=sh)
+    best, _ = autotune(ens, sides, log=log, stream=sh)
+    ens.start(sh)
+    ens.advance(steps - 1, use_graph=True, stream=sh)     # warm-up, graph capture
+    ens.upload(init, sh)
+    ens.start(sh)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    ens.advance(steps, use_graph=True, stream=sh)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_batch = e0.elapsed_time(e1) / 1e3
+    st = ens.status(sh)
+    assert all(s.err_kind == 0 and s.num_steps == steps + 1 for s in st), "ensemble run failed"
+    ens.close()
+
+    singles = []
+    for c, (W, U) in zip(cfgs, init):
+        m = gpu.MH2D(lib, c)
+        m.upload(W, U, stream=sh)
+        m.start(stream=sh)
+        # warm-up and graph capture; after 1 + steps launches (steps % 3 == 0)
+        # the timed replays use the captured controller phase
+        m.advance(steps, use_graph=True, stream=sh)
+        singles.append(m)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for m in singles:
+        m.advance(steps, use_graph=True, stream=sh)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_seq = e0.elapsed_time(e1) / 1e3
+    for m in singles:
+        m.close()
+    batch = cells * steps / t_batch
+    seq = cells * steps / t_seq
+    return {"instances": count, "cells": cells, "steps": steps, "strip_rule": f"{best[0]}={best[1]}",
+            "batched_cell_updates_per_s": batch, "sequential_cell_updates_per_s": seq,
+            "speedup": batch / seq, "batched_ms_per_step": t_batch / steps * 1e3,
+            "sequential_ms_per_step": t_seq / steps * 1e3}
