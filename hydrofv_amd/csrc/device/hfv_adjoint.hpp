@@ -24,7 +24,7 @@ enum AdjFlag : int { ADJ_OK = 0, ADJ_VACUUM = 1, ADJ_INVALID = 2 };
 // HLLC (2D, sweep-local state (rho, u_n, u_t, p)).  mf = cotangent of the
 // flux (mass, normal momentum, transverse momentum, energy); ml, mr receive
 // the cotangents of the left and right states (overwritten).
-__device__ __noinline__ int hllc_vjp(double gamma, double tol, const double* wl, const double* wr, const double* mf,
+HFV_HD __forceinline__ int hllc_vjp(double gamma, double tol, const double* wl, const double* wr, const double* mf,
                                      double* ml, double* mr)
 {
     for (int k = 0; k < 4; k++) ml[k] = mr[k] = 0.0;
@@ -248,7 +248,7 @@ __device__ __noinline__ int hllc_vjp(double gamma, double tol, const double* wl,
 // quirk = the density row uses the transverse slope (the y-sweep form).
 // Given cotangents mlo, mhi of the face states, add the cotangents of s
 // (ms) and of the slopes (md).  rho a^2 = gamma p is used for the pressure row.
-__device__ __forceinline__ void predict_vjp(const double* s, const double* d, double c, double gamma, bool quirk,
+HFV_HD __forceinline__ void predict_vjp(const double* s, const double* d, double c, double gamma, bool quirk,
                                             const double* mlo, const double* mhi, double* ms, double* md)
 {
     const double rho = s[0], n = s[1], p = s[3];
@@ -267,15 +267,39 @@ __device__ __forceinline__ void predict_vjp(const double* s, const double* d, do
     else md[1] += -0.5 * c * rho * m0;
 }
 
-// minmod selection: 0 -> zero slope, 1 -> the left difference, 2 -> the right one
-__device__ __forceinline__ int minmod_sel(double a, double b)
+// Partial derivatives (da, db) of limit<LIM>(a, b) (slope_limiter.c:49-91),
+// one-sided at the kinks with the branch the forward evaluation takes.
+template <int LIM>
+HFV_HD __forceinline__ void limit_partials(double a, double b, double* da, double* db)
 {
-    if (a * b <= 0.0) return 0;
-    return (fabs(a) < fabs(b)) ? 1 : 2;
+    *da = 0.0;
+    *db = 0.0;
+    if (a * b <= 0.0) return;
+    if constexpr (LIM == MINMOD) {
+        if (fabs(a) < fabs(b)) *da = 1.0;
+        else *db = 1.0;
+    } else if constexpr (LIM == VAN_LEER) {
+        // 2ab / (a + b + 1e-10)
+        const double s = a + b + 1e-10;
+        *da = 2.0 * b * (b + 1e-10) / (s * s);
+        *db = 2.0 * a * (a + 1e-10) / (s * s);
+    } else {
+        // MC: the central difference, or twice the smaller one-sided slope
+        const double cen = (a > 0) ? 0.5 * (a + b) : -0.5 * (a + b);
+        const double m2 = 2.0 * fmin(fabs(a), fabs(b));
+        if (cen <= m2) {
+            *da = 0.5;
+            *db = 0.5;
+        } else if (fabs(a) <= fabs(b)) {
+            *da = 2.0;
+        } else {
+            *db = 2.0;
+        }
+    }
 }
 
 // c2p (system.c:763-802) transposed: cotangent of (rho, u, v, p) -> of (m, mx, my, E)
-__device__ __forceinline__ void c2p_vjp(double gamma, double vol, double m, const double* w, const double* mw,
+HFV_HD __forceinline__ void c2p_vjp(double gamma, double vol, double m, const double* w, const double* mw,
                                         double* mu)
 {
     const double u = w[1], v = w[2];

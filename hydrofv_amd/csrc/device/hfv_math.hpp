@@ -1,5 +1,6 @@
 // hfv_math.hpp -- per-cell and per-interface physics for the MUSCL-Hancock
-// update, as __device__ functions for gfx950.
+// update, as functions for gfx950 kernels.  They are also compiled for the
+// host (HFV_HD), so that CPU tests can run the very same source.
 //
 // Every expression keeps the operand order of the reference C code it cites,
 // and the library is compiled with -ffp-contract=off, so each result is the
@@ -12,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+
+#define HFV_HD __host__ __device__
 
 namespace hfv {
 
@@ -32,7 +35,7 @@ enum ErrKind : int {
 // ---------------------------------------------------------------------------
 // slope limiters: slope_limiter.c:49-91
 template <int LIM>
-__device__ __forceinline__ double limit(double a, double b)
+HFV_HD __forceinline__ double limit(double a, double b)
 {
     if (a * b <= 0.0) return 0.0;
     if constexpr (LIM == MINMOD) {
@@ -45,7 +48,7 @@ __device__ __forceinline__ double limit(double a, double b)
     }
 }
 
-__device__ __forceinline__ double limit_rt(int lim, double a, double b)
+HFV_HD __forceinline__ double limit_rt(int lim, double a, double b)
 {
     switch (lim) {
     case MINMOD: return limit<MINMOD>(a, b);
@@ -55,20 +58,20 @@ __device__ __forceinline__ double limit_rt(int lim, double a, double b)
 }
 
 // utils.c:23-26
-__device__ __forceinline__ double sound_speed(double gamma, double rho, double p)
+HFV_HD __forceinline__ double sound_speed(double gamma, double rho, double p)
 {
     return sqrt(gamma * p / rho);
 }
 
 // riemann_solver.c:144-152
-__device__ __forceinline__ double coeff_A(double gamma, double rho) { return 2.0 / ((gamma + 1.0) * rho); }
-__device__ __forceinline__ double coeff_B(double gamma, double p) { return p * ((gamma - 1.0) / (gamma + 1.0)); }
+HFV_HD __forceinline__ double coeff_A(double gamma, double rho) { return 2.0 / ((gamma + 1.0) * rho); }
+HFV_HD __forceinline__ double coeff_B(double gamma, double p) { return p * ((gamma - 1.0) / (gamma + 1.0)); }
 
 // riemann_solver.c:154-224: PVRS / two-rarefaction / two-shock pressure guess,
 // in two parts.  guess_select returns true on the two-rarefaction branch
 // (188-199) with *g = its numerator al + ar - h*(ur - ul); otherwise *g is the
 // PVRS or two-shock guess before the clamp to tol.
-__device__ __forceinline__ bool guess_select(double gamma, double rl, double ul, double pl, double al,
+HFV_HD __forceinline__ bool guess_select(double gamma, double rl, double ul, double pl, double al,
                                              double rr, double ur, double pr, double ar, double* g)
 {
     const double plo = (pl > pr) ? pr : pl;
@@ -91,12 +94,12 @@ __device__ __forceinline__ bool guess_select(double gamma, double rl, double ul,
 
 // Two-rarefaction guess from its numerator and pow(pl, h/gamma), pow(pr, h/gamma);
 // the caller raises the result to gamma/h (riemann_solver.c:191-199).
-__device__ __forceinline__ double trrs_base(double num, double al, double ar, double pow_l, double pow_r)
+HFV_HD __forceinline__ double trrs_base(double num, double al, double ar, double pow_l, double pow_r)
 {
     return num / (al / (pow_l) + ar / (pow_r));
 }
 
-__device__ __forceinline__ double guess_p(double gamma, double rl, double ul, double pl, double al,
+HFV_HD __forceinline__ double guess_p(double gamma, double rl, double ul, double pl, double al,
                                           double rr, double ur, double pr, double ar, double tol)
 {
     double guess;
@@ -124,7 +127,7 @@ struct Prim {
 // the right-vacuum case; the left case's expressions are the right case's
 // with a, the front offset and the (u - speed) term negated, and negation is
 // exact, so both give the bits of the reference's separate branches.
-__device__ __forceinline__ Prim vac_fan(double gamma, Prim w, double a, double speed, bool right)
+HFV_HD __forceinline__ Prim vac_fan(double gamma, Prim w, double a, double speed, bool right)
 {
     const bool outside = right ? (speed <= (w.u - a)) : (speed >= (w.u + a));
     if (outside) return w;
@@ -143,7 +146,7 @@ __device__ __forceinline__ Prim vac_fan(double gamma, Prim w, double a, double s
     return o;
 }
 
-__device__ __forceinline__ Prim vacuum(double gamma, double rl, double ul, double vl, double pl, double al, double rr,
+HFV_HD __forceinline__ Prim vacuum(double gamma, double rl, double ul, double vl, double pl, double al, double rr,
                                        double ur, double vr, double pr, double ar, double speed)
 {
     // riemann_solver.c:377-481: which fan (if any) x/t = speed falls in
@@ -168,7 +171,7 @@ __device__ __forceinline__ Prim vacuum(double gamma, double rl, double ul, doubl
 
 // vacuum() on NF-component state arrays
 template <int NF>
-__device__ __forceinline__ Prim vacuum_of(double gamma, const double* wl, double al, const double* wr, double ar,
+HFV_HD __forceinline__ Prim vacuum_of(double gamma, const double* wl, double al, const double* wr, double ar,
                                           double speed)
 {
     const double vl = (NF == 4) ? wl[2] : 0.0;
@@ -180,7 +183,7 @@ __device__ __forceinline__ Prim vacuum_of(double gamma, const double* wl, double
 // HLLC: riemann_solver_hllc.c:30-376.  State order (rho, u_n, u_t, p); the
 // flux is (mass, normal momentum, transverse momentum, energy).  NF = 3 drops
 // the transverse component (1D).  Returns ERR_NONE or ERR_HLLC_SPEED.
-__device__ __forceinline__ double hllc_q(double gamma, double pk, double ps)
+HFV_HD __forceinline__ double hllc_q(double gamma, double pk, double ps)
 {
     if (ps <= pk) return 1.0;
     return sqrt(1.0 + (0.5 + 0.5 / gamma) * (ps / pk - 1.0));
@@ -188,7 +191,7 @@ __device__ __forceinline__ double hllc_q(double gamma, double pk, double ps)
 
 // Flux of the vacuum branch (hllc.c:42-66 via riemann_solver.c:377-481).
 template <int NF>
-__device__ __forceinline__ void hllc_vacuum_flux(double gamma, const double* wl, double al, const double* wr,
+HFV_HD __forceinline__ void hllc_vacuum_flux(double gamma, const double* wl, double al, const double* wr,
                                                  double ar, double* f)
 {
     const Prim v = vacuum_of<NF>(gamma, wl, al, wr, ar, 0.0);
@@ -209,7 +212,7 @@ __device__ __forceinline__ void hllc_vacuum_flux(double gamma, const double* wl,
 // Wave speeds and the flux of the region x/t = 0 falls in, given the
 // pressure guess ps (hllc.c:68-371).
 template <int NF>
-__device__ __forceinline__ int hllc_from_guess(double gamma, const double* wl, const double* wr, double al,
+HFV_HD __forceinline__ int hllc_from_guess(double gamma, const double* wl, const double* wr, double al,
                                                double ar, double ps, double* f)
 {
     const double rl = wl[0], ul = wl[1], pl = wl[NF - 1];
@@ -264,14 +267,14 @@ __device__ __forceinline__ int hllc_from_guess(double gamma, const double* wl, c
     return ERR_NONE;
 }
 
-__device__ __forceinline__ bool hllc_is_vacuum(double gamma, double rl, double ul, double al, double rr, double ur,
+HFV_HD __forceinline__ bool hllc_is_vacuum(double gamma, double rl, double ul, double al, double rr, double ur,
                                                double ar)
 {
     return rl <= 0.0 || rr <= 0.0 || ((al + ar) * 2.0 / (gamma + 1.0)) <= (ur - ul);
 }
 
 template <int NF>
-__device__ __forceinline__ int flux_hllc(double gamma, double tol, const double* wl, const double* wr, double* f)
+HFV_HD __forceinline__ int flux_hllc(double gamma, double tol, const double* wl, const double* wr, double* f)
 {
     const double rl = wl[0], ul = wl[1], pl = wl[NF - 1];
     const double rr = wr[0], ur = wr[1], pr = wr[NF - 1];
@@ -300,7 +303,7 @@ struct HllcFace {
     int kind;
 };
 
-__device__ __forceinline__ void hllc_begin(double gamma, double tol, const double* wl, const double* wr, HllcFace& F)
+HFV_HD __forceinline__ void hllc_begin(double gamma, double tol, const double* wl, const double* wr, HllcFace& F)
 {
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -327,7 +330,7 @@ __device__ __forceinline__ void hllc_begin(double gamma, double tol, const doubl
     F.ps = g;
 }
 
-__device__ __forceinline__ int hllc_end(double gamma, const HllcFace& F, double* f)
+HFV_HD __forceinline__ int hllc_end(double gamma, const HllcFace& F, double* f)
 {
     if (F.kind == HLLC_VACUUM) {
         hllc_vacuum_flux<4>(gamma, F.wl, F.al, F.wr, F.ar, f);
@@ -339,14 +342,14 @@ __device__ __forceinline__ int hllc_end(double gamma, const HllcFace& F, double*
 // ---------------------------------------------------------------------------
 // Exact Riemann solver: riemann_solver_exact.c:332-1080 (NEWTON 100 /
 // BRACKET 100 / BISECTION 5000 iterations, lines 26-28).
-__device__ __forceinline__ double wave_f(double gamma, double rho, double p, double a, double ps)
+HFV_HD __forceinline__ double wave_f(double gamma, double rho, double p, double a, double ps)
 {
     if (ps > p) return (ps - p) * sqrt(coeff_A(gamma, rho) / (ps + coeff_B(gamma, p)));
     return 2.0 * a * (pow((ps / p), (gamma - 1.0) / (2.0 * gamma)) - 1.0) / (gamma - 1.0);
 }
 
 // exact.c:539-565 -- the shock branch's (B - p*) is kept as written (553)
-__device__ __forceinline__ double wave_df(double gamma, double rho, double p, double a, double ps)
+HFV_HD __forceinline__ double wave_df(double gamma, double rho, double p, double a, double ps)
 {
     if (ps > p) {
         const double A = coeff_A(gamma, rho);
@@ -361,7 +364,7 @@ struct Side {
     double r, u, p, a;
 };
 
-__device__ __forceinline__ double pressure_fn(double gamma, Side L, Side R, double ps)
+HFV_HD __forceinline__ double pressure_fn(double gamma, Side L, Side R, double ps)
 {
     return wave_f(gamma, L.r, L.p, L.a, ps) + wave_f(gamma, R.r, R.p, R.a, ps) + (R.u - L.u);
 }
@@ -371,7 +374,7 @@ struct PStar {
     int rc;
 };
 
-__device__ __noinline__ PStar solve_p_star(double gamma, Side L, Side R, double tol)
+HFV_HD inline __noinline__ PStar solve_p_star(double gamma, Side L, Side R, double tol)
 {
     const double pg = guess_p(gamma, L.r, L.u, L.p, L.a, R.r, R.u, R.p, R.a, tol);
     bool bracketed = false;
@@ -437,7 +440,7 @@ __device__ __noinline__ PStar solve_p_star(double gamma, Side L, Side R, double 
 
 // exact.c:765-974 (left), 817-1080 (right; note its own operand orders).
 // Returns (rho, u, p) in Prim's r, u, p.
-__device__ __forceinline__ Prim sample_left(double gamma, Side W, double us, double ps, double speed)
+HFV_HD __forceinline__ Prim sample_left(double gamma, Side W, double us, double ps, double speed)
 {
     const double rho = W.r, u = W.u, p = W.p, a = W.a;
     Prim o{rho, u, 0.0, p};
@@ -468,7 +471,7 @@ __device__ __forceinline__ Prim sample_left(double gamma, Side W, double us, dou
     return o;
 }
 
-__device__ __forceinline__ Prim sample_right(double gamma, Side W, double us, double ps, double speed)
+HFV_HD __forceinline__ Prim sample_right(double gamma, Side W, double us, double ps, double speed)
 {
     const double rho = W.r, u = W.u, p = W.p, a = W.a;
     Prim o{rho, u, 0.0, p};
@@ -509,7 +512,7 @@ struct Sample {
     int rc;
 };
 
-__device__ __noinline__ Sample exact_sample(double gamma, double tol, double rl, double ul, double vl, double pl,
+HFV_HD inline __noinline__ Sample exact_sample(double gamma, double tol, double rl, double ul, double vl, double pl,
                                             double rr, double ur, double vr, double pr, double speed)
 {
     const double al = (rl > 0.0) ? sound_speed(gamma, rl, pl) : 0.0;
@@ -548,7 +551,7 @@ __device__ __noinline__ Sample exact_sample(double gamma, double tol, double rl,
 // exact.c:431, else v_R); on the vacuum branch the vacuum sampler carries it.
 // At v = 0 the 2D flux equals the 1D flux bitwise.
 template <int NF>
-__device__ __forceinline__ int flux_exact(double gamma, double tol, const double* wl, const double* wr, double* f)
+HFV_HD __forceinline__ int flux_exact(double gamma, double tol, const double* wl, const double* wr, double* f)
 {
     const double vl = (NF == 4) ? wl[2] : 0.0;
     const double vr = (NF == 4) ? wr[2] : 0.0;
@@ -567,7 +570,7 @@ __device__ __forceinline__ int flux_exact(double gamma, double tol, const double
 }
 
 template <int SOLVER, int NF>
-__device__ __forceinline__ int flux(double gamma, double tol, const double* wl, const double* wr, double* f)
+HFV_HD __forceinline__ int flux(double gamma, double tol, const double* wl, const double* wr, double* f)
 {
     if constexpr (SOLVER == HLLC) return flux_hllc<NF>(gamma, tol, wl, wr, f);
     else return flux_exact<NF>(gamma, tol, wl, wr, f);
@@ -579,7 +582,7 @@ __device__ __forceinline__ int flux(double gamma, double tol, const double* wl, 
 // slopes, c = dt*A/V, a = sound speed.  lo = state at the cell's low face,
 // hi = state at its high face.  The y-sweep's density row multiplies the
 // x-velocity slope (977-982, 1002-1007); that form is kept.
-__device__ __forceinline__ void predict_x(const double* s, const double* d, double c, double a, double* lo, double* hi)
+HFV_HD __forceinline__ void predict_x(const double* s, const double* d, double c, double a, double* lo, double* hi)
 {
     const double up = 1.0 + c * s[1];
     const double um = 1.0 - c * s[1];
@@ -593,7 +596,7 @@ __device__ __forceinline__ void predict_x(const double* s, const double* d, doub
     hi[3] = s[3] + 0.5 * (0.0 - c * s[0] * a * a * d[1] + 0.0 + um * d[3]);
 }
 
-__device__ __forceinline__ void predict_y(const double* s, const double* d, double c, double a, double* lo, double* hi)
+HFV_HD __forceinline__ void predict_y(const double* s, const double* d, double c, double a, double* lo, double* hi)
 {
     const double vp = 1.0 + c * s[2];
     const double vm = 1.0 - c * s[2];
@@ -607,7 +610,7 @@ __device__ __forceinline__ void predict_y(const double* s, const double* d, doub
     hi[3] = s[3] + 0.5 * (0.0 + 0.0 - c * s[0] * a * a * d[2] + vm * d[3]);
 }
 
-__device__ __forceinline__ void predict_1d(const double* s, const double* d, double c, double a, double* lo, double* hi)
+HFV_HD __forceinline__ void predict_1d(const double* s, const double* d, double c, double a, double* lo, double* hi)
 {
     lo[0] = s[0] + -0.5 * ((1.0 + c * s[1]) * d[0] + c * s[0] * d[1] + 0.0);
     lo[1] = s[1] + -0.5 * (0.0 + (1.0 + c * s[1]) * d[1] + (c / s[0]) * d[2]);
@@ -619,7 +622,7 @@ __device__ __forceinline__ void predict_1d(const double* s, const double* d, dou
 
 // ---------------------------------------------------------------------------
 // Conversions: system.c:763-802 (2D c2p), 735-761 (1D c2p).
-__device__ __forceinline__ void c2p_2d(double gamma, double vol, double m, double mx, double my, double e, double* w)
+HFV_HD __forceinline__ void c2p_2d(double gamma, double vol, double m, double mx, double my, double e, double* w)
 {
     w[0] = m / vol;
     w[1] = mx / m;
@@ -627,7 +630,7 @@ __device__ __forceinline__ void c2p_2d(double gamma, double vol, double m, doubl
     w[3] = (gamma - 1.0) * (e - 0.5 * m * (w[1] * w[1] + w[2] * w[2])) / vol;
 }
 
-__device__ __forceinline__ void c2p_1d(double gamma, double vol, double m, double mx, double e, double* w)
+HFV_HD __forceinline__ void c2p_1d(double gamma, double vol, double m, double mx, double e, double* w)
 {
     w[0] = m / vol;
     w[1] = mx / m;
@@ -637,12 +640,14 @@ __device__ __forceinline__ void c2p_1d(double gamma, double vol, double m, doubl
 // CFL reduction key: the maximum of non-negative doubles as an unsigned
 // maximum of (bit pattern + 1); key 0 stands for "no finite candidate",
 // i.e. the reference's initial -INFINITY (utils.c:75-92).
-__device__ __forceinline__ unsigned long long smax_key(double v)
+HFV_HD __forceinline__ unsigned long long smax_key(double v)
 {
-    return (v >= 0.0) ? (unsigned long long)__double_as_longlong(v) + 1ull : 0ull;
+    union { double d; unsigned long long u; } c;
+    c.d = v;
+    return (v >= 0.0) ? c.u + 1ull : 0ull;
 }
 
-__host__ __device__ __forceinline__ double smax_from_key(unsigned long long k)
+__host__ HFV_HD __forceinline__ double smax_from_key(unsigned long long k)
 {
     if (k == 0ull) return -INFINITY;
     union { unsigned long long u; double d; } c;

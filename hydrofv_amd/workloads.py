@@ -116,3 +116,35 @@ def ensemble_specs(count: int = 256, seed: int = 2, side_lo: int = 64, side_hi: 
     states[..., 2] = -1.0 + 2.0 * r[..., 2]
     states[..., 3] = -1.0 + 2.0 * r[..., 3]
     return sides, states
+
+
+def normal01(seed: int, n: int) -> np.ndarray:
+    """n standard normal deviates from one splitmix64 stream (Box-Muller)."""
+    u = uniform01(seed, 2 * ((n + 1) // 2)).reshape(2, -1)
+    r = np.sqrt(-2.0 * np.log(1.0 - u[0]))   # 1 - u in (0, 1]
+    z = np.concatenate([r * np.cos(2.0 * math.pi * u[1]), r * np.sin(2.0 * math.pi * u[1])])
+    return z[:n]
+
+
+def smooth_field(n: int, seed: int, sigma: float = 16.0) -> np.ndarray:
+    """White noise on an n x n periodic grid smoothed by a Gaussian of
+    `sigma` cells (via FFT), scaled to max |f| = 1."""
+    z = normal01(seed, n * n).reshape(n, n)
+    k = np.fft.fftfreq(n)
+    kk = k[:, None] ** 2 + k[None, :] ** 2
+    f = np.fft.ifft2(np.fft.fft2(z) * np.exp(-2.0 * (math.pi * sigma) ** 2 * kk)).real
+    return f / np.abs(f).max()
+
+
+def adjoint_initial(n: int, seed: int = 0, sigma: float = 16.0):
+    """Primitive interior state (rho, u, v, p) of the adjoint workload:
+    rho = 1 + 0.5 f1, p = 1 + 0.5 f2 (both in [0.5, 1.5]), (u, v) = 0.3
+    (f3, f4) / max |(f3, f4)|, from smooth fields of seeds 3..6 (+ 100 seed)."""
+    f = [smooth_field(n, 3 + k + 100 * seed, sigma) for k in range(4)]
+    speed = np.sqrt(f[2] ** 2 + f[3] ** 2).max()
+    return np.stack([1.0 + 0.5 * f[0], 0.3 * f[2] / speed, 0.3 * f[3] / speed, 1.0 + 0.5 * f[1]])
+
+
+def adjoint_perturbation(n: int, seed: int = 0, sigma: float = 16.0, amplitude: float = 1e-2) -> np.ndarray:
+    """Density perturbation of the target run: amplitude * f5 (seed 7 + 100 seed)."""
+    return amplitude * smooth_field(n, 7 + 100 * seed, sigma)

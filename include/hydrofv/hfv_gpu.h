@@ -124,6 +124,37 @@ typedef struct hfv_mh1d_config {
 int hfv_mh1d_run(const hfv_mh1d_config *cfg, double *const W[3], double *const U[3], const double *area,
                  const double *mid, const double *volume, hfv_run_status *st, void *stream);
 
+/* ---- fixed-dt 2D step and its discrete adjoint (no reference symbol) ---- */
+/* The 2D MUSCL-Hancock + HLLC step of integrator_muscl_hancock.c:552-1141
+ * with a given dt on a doubly periodic Cartesian grid, as a stateless
+ * operation on caller-owned device arrays (4 conserved fields, each
+ * (nx+2g) columns x (ny+2g) rows with row pitch `pitch` doubles), and the
+ * transposed Jacobian of that step.  The state is the interior; ghost
+ * entries are neither read nor given gradient.  Neither call allocates; both
+ * are asynchronous on `stream` (0: the null stream) and OR HFV_ADJ_* bits
+ * into the device int *flags, which the caller clears and reads. */
+typedef struct hfv_adj2d_config {
+    int num_cells_x, num_cells_y, num_ghost_cells_side;
+    long long pitch;               /* row pitch of every field, in doubles (>= nx + 2g) */
+    double gamma, dt, tol;
+    double area_x, area_y, volume; /* uniform Cartesian geometry (system.c:400-415, 490-503) */
+    int slope_limiter;             /* slope_limiter.h flags: minmod 1, Van Leer 2, MC 3 */
+} hfv_adj2d_config;
+
+enum {
+    HFV_ADJ_FLUX_ERROR = 1,  /* forward: an HLLC flux failed (invalid wave speeds) */
+    HFV_ADJ_VACUUM = 2,      /* adjoint: a face took the vacuum branch; its derivative is taken as 0 */
+    HFV_ADJ_INVALID = 4      /* adjoint: invalid HLLC wave speeds at a face */
+};
+
+/* V = step(U); U and V must not overlap. */
+int hfv_fwd2d_step(const hfv_adj2d_config *cfg, double *const U[4], double *const V[4], int *flags, void *stream);
+/* lam_out = lam_in + (d step(U) / dU)^T lam_in on the interior (lam_in: the
+ * cotangent of step(U), lam_out: that of U); lam_out must not overlap U or
+ * lam_in, and its ghost entries are left untouched. */
+int hfv_adj2d_step(const hfv_adj2d_config *cfg, double *const U[4], double *const lam_in[4],
+                   double *const lam_out[4], int *flags, void *stream);
+
 /* ---- batched evaluation of the device physics (parity probes) ---------- */
 /* f[n][nf] = flux of (wl[n][nf], wr[n][nf]); nf 3 (1D) or 4 (2D); err[n]. */
 int hfv_probe_flux(int nf, int solver, double gamma, double tol, long long n, const double *wl,

@@ -1,0 +1,111 @@
+// hostcheck.cpp -- the device math of hydrofv_amd (hfv_math.hpp,
+// hfv_adjoint.hpp) compiled for the host, with batched C entry points, so
+// that CPU tests can check the kernels' own source: forward pieces against
+// the oracle, derivative pieces against automatic differentiation.
+// Test infrastructure only; built by tests/native/Makefile.
+#include "../../hydrofv_amd/csrc/device/hfv_adjoint.hpp"
+
+using namespace hfv;
+
+namespace {
+
+template <int LIM>
+void limits(long long n, const double* a, const double* b, double* d, double* da, double* db)
+{
+    for (long long i = 0; i < n; i++) {
+        d[i] = limit<LIM>(a[i], b[i]);
+        limit_partials<LIM>(a[i], b[i], &da[i], &db[i]);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// f[n][nf] = HLLC flux (flux_hllc<nf>); err[n]
+void hc_flux_hllc(int nf, double gamma, double tol, long long n, const double* wl, const double* wr, double* f,
+                  int* err)
+{
+    for (long long i = 0; i < n; i++) {
+        if (nf == 3) err[i] = flux_hllc<3>(gamma, tol, wl + 3 * i, wr + 3 * i, f + 3 * i);
+        else err[i] = flux_hllc<4>(gamma, tol, wl + 4 * i, wr + 4 * i, f + 4 * i);
+    }
+}
+
+// The 2D kernels' split form: hllc_begin, the two-rarefaction powers taken
+// as the wavefront takes them, hllc_end.
+void hc_flux_hllc_split(double gamma, double tol, long long n, const double* wl, const double* wr, double* f,
+                        int* err)
+{
+    for (long long i = 0; i < n; i++) {
+        HllcFace F;
+        hllc_begin(gamma, tol, wl + 4 * i, wr + 4 * i, F);
+        if (F.kind == HLLC_TRRS) {
+            const double h = 0.5 * (gamma - 1.0);
+            const double pl = pow(F.wl[3], h / gamma), pr = pow(F.wr[3], h / gamma);
+            F.ps = pow(trrs_base(F.ps, F.al, F.ar, pl, pr), gamma / h);
+            if (F.ps < tol) F.ps = tol;
+            F.kind = HLLC_READY;
+        }
+        err[i] = hllc_end(gamma, F, f + 4 * i);
+    }
+}
+
+void hc_flux_exact(int nf, double gamma, double tol, long long n, const double* wl, const double* wr, double* f,
+                   int* err)
+{
+    for (long long i = 0; i < n; i++) {
+        if (nf == 3) err[i] = flux_exact<3>(gamma, tol, wl + 3 * i, wr + 3 * i, f + 3 * i);
+        else err[i] = flux_exact<4>(gamma, tol, wl + 4 * i, wr + 4 * i, f + 4 * i);
+    }
+}
+
+// ml, mr [n][4]: cotangents of the HLLC input states for flux cotangent mf
+void hc_hllc_vjp(double gamma, double tol, long long n, const double* wl, const double* wr, const double* mf,
+                 double* ml, double* mr, int* flag)
+{
+    for (long long i = 0; i < n; i++)
+        flag[i] = hllc_vjp(gamma, tol, wl + 4 * i, wr + 4 * i, mf + 4 * i, ml + 4 * i, mr + 4 * i);
+}
+
+// x (y = 0) or y (y = 1) predictor in the global order (rho, u, v, p)
+void hc_predict(int y, long long n, const double* s, const double* d, double c, double gamma, double* lo,
+                double* hi)
+{
+    for (long long i = 0; i < n; i++) {
+        const double* w = s + 4 * i;
+        const double a = sound_speed(gamma, w[0], w[3]);
+        if (y) predict_y(w, d + 4 * i, c, a, lo + 4 * i, hi + 4 * i);
+        else predict_x(w, d + 4 * i, c, a, lo + 4 * i, hi + 4 * i);
+    }
+}
+
+// predictor VJP in sweep-local order; quirk = the y-sweep density row
+void hc_predict_vjp(int quirk, long long n, const double* s, const double* d, double c, double gamma,
+                    const double* mlo, const double* mhi, double* ms, double* md)
+{
+    for (long long i = 0; i < n; i++) {
+        for (int k = 0; k < 4; k++) ms[4 * i + k] = 0.0;
+        predict_vjp(s + 4 * i, d + 4 * i, c, gamma, quirk != 0, mlo + 4 * i, mhi + 4 * i, ms + 4 * i, md + 4 * i);
+    }
+}
+
+void hc_limit(int lim, long long n, const double* a, const double* b, double* d, double* da, double* db)
+{
+    if (lim == MINMOD) limits<MINMOD>(n, a, b, d, da, db);
+    else if (lim == VAN_LEER) limits<VAN_LEER>(n, a, b, d, da, db);
+    else limits<MC>(n, a, b, d, da, db);
+}
+
+// mu [n][4] = J_c2p(U)^T mw, U = (m, mx, my, E)
+void hc_c2p_vjp(double gamma, double vol, long long n, const double* U, const double* mw, double* mu)
+{
+    for (long long i = 0; i < n; i++) {
+        const double* u = U + 4 * i;
+        double w[4];
+        c2p_2d(gamma, vol, u[0], u[1], u[2], u[3], w);
+        c2p_vjp(gamma, vol, u[0], w, mw + 4 * i, mu + 4 * i);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,235 @@
+"""CPU checks of the device math's own source (compiled for the host by
+tests/native): forward pieces against the oracle bit for bit, and the
+hand-written derivatives of the discrete adjoint (hfv_adjoint.hpp) against
+torch float64 autograd of the restatement in _torch_ref.py.  Also pins that
+restatement's step to the oracle's fixed-dt step."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import _cases as K
+import _oracle as O
+import _torch_ref as T
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HOSTCHECK_SO = os.path.join(HERE, "native", "libhfv_hostcheck.so")
+GAMMA, TOL = 1.4, 1e-6
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+@pytest.fixture(scope="module")
+def hc():
+    if not os.path.exists(HOSTCHECK_SO):
+        pytest.fail(f"{HOSTCHECK_SO} is not built (make -C tests/native)")
+    L = C.CDLL(HOSTCHECK_SO)
+    L.hc_flux_hllc.argtypes = [C.c_int, C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp, _ip]
+    L.hc_flux_hllc_split.argtypes = [C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp, _ip]
+    L.hc_flux_exact.argtypes = [C.c_int, C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp, _ip]
+    L.hc_hllc_vjp.argtypes = [C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp, _dp, _dp, _ip]
+    L.hc_predict.argtypes = [C.c_int, C.c_longlong, _dp, _dp, C.c_double, C.c_double, _dp, _dp]
+    L.hc_predict_vjp.argtypes = [C.c_int, C.c_longlong, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+    L.hc_limit.argtypes = [C.c_int, C.c_longlong, _dp, _dp, _dp, _dp, _dp]
+    L.hc_c2p_vjp.argtypes = [C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp]
+    for f in ("hc_flux_hllc", "hc_flux_hllc_split", "hc_flux_exact", "hc_hllc_vjp", "hc_predict",
+              "hc_predict_vjp", "hc_limit", "hc_c2p_vjp"):
+        getattr(L, f).restype = None
+    return L
+
+
+def rng_states(n, seed, rho=(0.1, 2.0), vel=1.5, p=(0.05, 3.0)):
+    r = np.random.default_rng(seed)
+    w = np.empty((n, 4))
+    w[:, 0] = r.uniform(*rho, n)
+    w[:, 1] = r.uniform(-vel, vel, n)
+    w[:, 2] = r.uniform(-vel, vel, n)
+    w[:, 3] = r.uniform(*p, n)
+    return w
+
+
+# ---------------------------------------------------------------------------
+# forward pieces: the device source on the host equals the oracle bit for bit
+@pytest.mark.parametrize("nf", [3, 4])
+def test_host_hllc_equals_oracle(hc, nf):
+    wl, wr = K.riemann_states(20000, 11)
+    if nf == 3:
+        wl, wr = wl[:, [0, 1, 3]].copy(), wr[:, [0, 1, 3]].copy()
+    n = len(wl)
+    f = np.empty((n, nf))
+    err = np.empty(n, np.int32)
+    hc.hc_flux_hllc(nf, GAMMA, TOL, n, wl, wr, f, err)
+    fo, eo = O.flux_batch(2 if nf == 4 else 1, "riemann_solver_hllc", GAMMA, TOL, wl, wr)
+    ok = eo == 0
+    assert ok.sum() > n // 2
+    assert np.array_equal(f[ok], fo[ok], equal_nan=True)
+    assert np.array_equal(err[ok], np.zeros(ok.sum(), np.int32))
+    assert np.all(err[~ok] != 0)
+
+
+def test_split_hllc_equals_plain(hc):
+    """The kernels' split HLLC (begin, powers, end) is the same arithmetic."""
+    wl, wr = K.riemann_states(20000, 12)
+    n = len(wl)
+    f1, f2 = np.empty((n, 4)), np.empty((n, 4))
+    e1, e2 = np.empty(n, np.int32), np.empty(n, np.int32)
+    hc.hc_flux_hllc(4, GAMMA, TOL, n, wl, wr, f1, e1)
+    hc.hc_flux_hllc_split(GAMMA, TOL, n, wl, wr, f2, e2)
+    assert np.array_equal(e1, e2)
+    ok = e1 == 0  # on a failure the flux is left unwritten
+    assert np.array_equal(f1[ok], f2[ok], equal_nan=True)
+
+
+@pytest.mark.parametrize("nf", [3, 4])
+def test_host_exact_equals_oracle(hc, nf):
+    wl, wr = K.riemann_states(3000, 13)
+    if nf == 3:
+        wl, wr = wl[:, [0, 1, 3]].copy(), wr[:, [0, 1, 3]].copy()
+    n = len(wl)
+    f = np.empty((n, nf))
+    err = np.empty(n, np.int32)
+    hc.hc_flux_exact(nf, GAMMA, 1e-8, n, wl, wr, f, err)
+    fo, eo = O.flux_batch(2 if nf == 4 else 1, "riemann_solver_exact", GAMMA, 1e-8, wl, wr)
+    ok = eo == 0
+    assert ok.sum() > n // 2
+    assert np.array_equal(f[ok], fo[ok], equal_nan=True)
+
+
+# ---------------------------------------------------------------------------
+# derivatives against autograd
+@pytest.mark.parametrize("lim", [T.MINMOD, T.VAN_LEER, T.MC])
+def test_limiter_partials(hc, lim):
+    r = np.random.default_rng(lim)
+    a = r.normal(size=4000)
+    b = r.normal(size=4000)
+    b[:500] = -a[:500] * r.uniform(0.1, 2, 500)      # opposite signs
+    b[500:1000] = a[500:1000] * r.uniform(0.2, 5, 500)  # same sign, every ratio
+    d, da, db = np.empty_like(a), np.empty_like(a), np.empty_like(a)
+    hc.hc_limit(lim, len(a), a, b, d, da, db)
+    ta = torch.tensor(a, requires_grad=True)
+    tb = torch.tensor(b, requires_grad=True)
+    td = T.limit(lim, ta, tb)
+    ga, gb = torch.autograd.grad(td.sum(), (ta, tb))
+    np.testing.assert_allclose(d, td.detach().numpy(), rtol=1e-15, atol=0)
+    np.testing.assert_allclose(da, ga.numpy(), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(db, gb.numpy(), rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("y", [0, 1])
+def test_predictor_vjp(hc, y):
+    n = 3000
+    s = rng_states(n, 20 + y, vel=0.8)
+    r = np.random.default_rng(30 + y)
+    d = r.normal(scale=0.2, size=(n, 4))
+    c = 0.37
+    lo, hi = np.empty((n, 4)), np.empty((n, 4))
+    hc.hc_predict(y, n, s, d, c, GAMMA, lo, hi)
+    ts = torch.tensor(s.T.copy(), requires_grad=True)
+    td = torch.tensor(d.T.copy(), requires_grad=True)
+    a = T.sound_speed(GAMMA, ts[0], ts[3])
+    tlo, thi = (T.predict_y if y else T.predict_x)(ts, td, c, a)
+    # torch rounds a scalar-by-tensor division differently in the last place
+    np.testing.assert_allclose(lo, tlo.detach().numpy().T, rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(hi, thi.detach().numpy().T, rtol=1e-14, atol=1e-15)
+    mlo = r.normal(size=(n, 4))
+    mhi = r.normal(size=(n, 4))
+    gs, gd = torch.autograd.grad((tlo, thi), (ts, td), (torch.tensor(mlo.T.copy()), torch.tensor(mhi.T.copy())))
+    gs, gd = gs.numpy().T, gd.numpy().T
+    # the device VJP works in sweep-local order (rho, normal, transverse, p)
+    perm = [0, 2, 1, 3] if y else [0, 1, 2, 3]
+    ms, md = np.empty((n, 4)), np.empty((n, 4))
+    hc.hc_predict_vjp(y, n, s[:, perm].copy(), d[:, perm].copy(), c, GAMMA, mlo[:, perm].copy(),
+                      mhi[:, perm].copy(), ms, md)
+    np.testing.assert_allclose(ms[:, perm], gs, rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(md[:, perm], gd, rtol=1e-12, atol=1e-13)
+
+
+def test_c2p_vjp(hc):
+    n = 3000
+    w = rng_states(n, 40)
+    vol = 0.013
+    U = np.stack([w[:, 0] * vol, w[:, 0] * vol * w[:, 1], w[:, 0] * vol * w[:, 2],
+                  vol * (w[:, 3] / (GAMMA - 1) + 0.5 * w[:, 0] * (w[:, 1] ** 2 + w[:, 2] ** 2))], 1)
+    mw = np.random.default_rng(41).normal(size=(n, 4))
+    mu = np.empty((n, 4))
+    hc.hc_c2p_vjp(GAMMA, vol, n, U, mw, mu)
+    tU = torch.tensor(U.T.copy(), requires_grad=True)
+    (g,) = torch.autograd.grad(T.c2p(GAMMA, vol, tU), tU, torch.tensor(mw.T.copy()))
+    np.testing.assert_allclose(mu, g.numpy().T, rtol=1e-12, atol=1e-12 * np.abs(g.numpy()).max())
+
+
+def _hllc_cases(n, seed):
+    """Random face states over every branch: mild states (PVRS), strong
+    expansions (two-rarefaction), strong compressions (two-shock), fast
+    flows (one-sided regions)."""
+    r = np.random.default_rng(seed)
+    wl = rng_states(n, seed, vel=0.5)
+    wr = rng_states(n, seed + 1, vel=0.5)
+    k = n // 4
+    wl[k:2 * k, 1] = -r.uniform(0.2, 1.2, k)       # expansion
+    wr[k:2 * k, 1] = r.uniform(0.2, 1.2, k)
+    wl[2 * k:3 * k, 1] = r.uniform(0.5, 2.0, k)     # compression
+    wr[2 * k:3 * k, 1] = -r.uniform(0.5, 2.0, k)
+    sgn = np.where(r.uniform(size=n - 3 * k) < 0.5, -1.0, 1.0)
+    wl[3 * k:, 1] = sgn * r.uniform(1.5, 4.0, n - 3 * k)   # supersonic either way
+    wr[3 * k:, 1] = wl[3 * k:, 1] + r.normal(scale=0.2, size=n - 3 * k)
+    return wl, wr
+
+
+def test_hllc_vjp_against_autograd(hc):
+    n = 4000
+    wl, wr = _hllc_cases(n, 50)
+    mf = np.random.default_rng(51).normal(size=(n, 4))
+    ml, mr = np.empty((n, 4)), np.empty((n, 4))
+    flag = np.empty(n, np.int32)
+    hc.hc_hllc_vjp(GAMMA, TOL, n, wl, wr, mf, ml, mr, flag)
+    ok = flag == 0
+    tl = torch.tensor(wl[ok].T.copy(), requires_grad=True)
+    tr = torch.tensor(wr[ok].T.copy(), requires_grad=True)
+    f = T.hllc(GAMMA, TOL, tl, tr)
+    # the restatement's flux is the device flux
+    fd = np.empty((n, 4))
+    e = np.empty(n, np.int32)
+    hc.hc_flux_hllc(4, GAMMA, TOL, n, wl, wr, fd, e)
+    np.testing.assert_allclose(f.detach().numpy().T, fd[ok], rtol=1e-13, atol=1e-14)
+    gl, gr = torch.autograd.grad(f, (tl, tr), torch.tensor(mf[ok].T.copy()))
+    gl, gr = gl.numpy().T, gr.numpy().T
+    scale = np.maximum(np.abs(gl).max(1, keepdims=True), np.abs(gr).max(1, keepdims=True))
+    assert np.abs(ml[ok] - gl).max() / 1 <= 1e-10 * scale.max()
+    rel_l = np.abs(ml[ok] - gl) / scale
+    rel_r = np.abs(mr[ok] - gr) / scale
+    assert rel_l.max() < 1e-11 and rel_r.max() < 1e-11, (rel_l.max(), rel_r.max())
+    # branch coverage of the checked set
+    al = np.sqrt(GAMMA * wl[ok, 3] / wl[ok, 0])
+    ar = np.sqrt(GAMMA * wr[ok, 3] / wr[ok, 0])
+    ppv = 0.5 * (wl[ok, 3] + wr[ok, 3]) - 0.125 * (wr[ok, 1] - wl[ok, 1]) * (wl[ok, 0] + wr[ok, 0]) * (al + ar)
+    plo = np.minimum(wl[ok, 3], wr[ok, 3])
+    phi = np.maximum(wl[ok, 3], wr[ok, 3])
+    pvrs = (phi / plo <= 2) & (plo <= ppv) & (ppv <= phi)
+    trrs = ~pvrs & (ppv < plo)
+    assert pvrs.sum() > 100 and trrs.sum() > 100 and (~pvrs & ~trrs).sum() > 100
+    assert (flag == 1).sum() < n // 4
+
+
+def test_torch_step_matches_oracle():
+    """The restatement's step equals the oracle's fixed-dt periodic step to
+    rounding (the oracle is pinned to the reference build elsewhere)."""
+    import hydrofv_amd
+    lib = hydrofv_amd.load()
+    nx, ny, g = 14, 10, 2
+    sim = K.setup_random_2d(lib, nx, ny, g, ("periodic",) * 4, seed=5, gamma=GAMMA)
+    grid = K.oracle_grid(sim, "minmod")
+    dx, dy = sim.system.dx_, sim.system.dy_
+    U0 = np.stack(sim.conserved())[:, g:-g, g:-g].copy()
+    dt = 2e-3
+    r = grid.run(0.8, max_steps=2, fixed_dt=dt)
+    assert r.error == 0 and r.num_steps == 2
+    U = torch.tensor(U0)
+    for _ in range(2):
+        U = T.step(U, GAMMA, dt, dx, dy)
+    ref = grid.U[:, g:-g, g:-g]
+    err = np.abs(U.numpy() - ref).max(axis=(1, 2)) / np.abs(ref).max(axis=(1, 2))
+    assert err.max() < 1e-14, err
+    sim.close()
