@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--adjoint-solves", type=int, default=32,
+                    help="gradient solves of the 1024^2, 500-step adjoint workload (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=96,
                     help="steps of the 256-instance batched ensemble and of its sequential baseline (0: skip)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -183,6 +185,15 @@ def main():
         except Exception as e:  # the main figure stands without it
             log(f"ensemble benchmark failed: {e!r}")
 
+    adj = None
+    if args.adjoint_solves:
+        from hydrofv_amd import adjoint
+        try:
+            adj = adjoint.benchmark(n=1024, steps=500, solves=args.adjoint_solves, log=log)
+            log(f"adjoint: {adj}")
+        except Exception as e:  # the main figure stands without it
+            log(f"adjoint benchmark failed: {e!r}")
+
     traffic = measured_traffic()
     line = {
         "metric": "cell-updates/s FP64 MUSCL-HLLC 4096²; % HBM roofline; batched & adjoint ratio",
@@ -204,6 +215,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "adjoint": adj,
     }
     print(json.dumps(line), flush=True)
 

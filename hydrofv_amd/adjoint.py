@@ -187,3 +187,111 @@ def density_loss(U: torch.Tensor, rho_target: torch.Tensor, grid: Periodic2D) ->
     """1/2 sum over the interior of (rho - rho_target)^2, rho = m / V."""
     rho = grid.interior(U)[0] / grid.volume
     return 0.5 * ((rho - rho_target) ** 2).sum()
+
+
+def conserved_from_primitive(W, grid: Periodic2D, device="cuda") -> torch.Tensor:
+    """(rho, u, v, p) interior arrays -> U with zero ghost cells (m = rho V,
+    momenta m u and m v, E = V (p/(gamma-1) + rho |v|^2 / 2), system.c:804-843)."""
+    import numpy as np
+    rho, u, v, p = (np.asarray(x, dtype=np.float64) for x in W)
+    vol = grid.volume
+    m = rho * vol
+    U = torch.zeros(grid.shape(), dtype=torch.float64)
+    inner = grid.interior(U)
+    inner[0] = torch.from_numpy(m)
+    inner[1] = torch.from_numpy(m * u)
+    inner[2] = torch.from_numpy(m * v)
+    inner[3] = torch.from_numpy(vol * (p / (grid.gamma - 1.0) + 0.5 * rho * (u * u + v * v)))
+    return U.to(device)
+
+
+def cfl_time_step(W, dx, dy, gamma, cfl=0.8):
+    """The reference's CFL time step of a primitive state (utils.c:75-92)."""
+    import numpy as np
+    rho, u, v, p = W
+    a = np.sqrt(gamma * p / rho)
+    return cfl / ((a + np.abs(u)).max() / dx + (a + np.abs(v)).max() / dy)
+
+
+def workload_grid(W, n, gamma=1.4, g=2, cfl=0.8, limiter=1) -> Periodic2D:
+    """The unit square as the reference lays it out (dx = 1 / (n + 2g)) with
+    the CFL-0.8 time step of W kept fixed."""
+    dx = 1.0 / (n + 2 * g)
+    return Periodic2D(n, n, dx, dx, cfl_time_step(W, dx, dx, gamma, cfl), gamma=gamma, g=g, limiter=limiter)
+
+
+def benchmark(n=1024, steps=500, solves=32, sigma=16.0, log=None):
+    """Adjoint workload: minmod + HLLC, periodic n x n, smooth random initial
+    states, `steps` fixed-dt steps, density-mismatch loss against a target
+    run from a 1e-2 density perturbation; the gradient w.r.t. all initial
+    conserved variables by the checkpointed adjoint, for `solves` seeds.
+    Reports the kernel-level adjoint/forward step cost and the time per
+    gradient solve against a plain forward run."""
+    from . import workloads
+
+    dev = torch.device("cuda")
+    W = workloads.adjoint_initial(n, seed=0, sigma=sigma)
+    grid = workload_grid(W, n)
+    U0 = conserved_from_primitive(W, grid)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 100
+    a, b = U0.clone(), torch.empty_like(U0)
+    for _ in range(10):
+        grid.step(a, b)
+        a, b = b, a
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        grid.step(a, b)
+        a, b = b, a
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / reps
+    lam = torch.zeros_like(U0)
+    grid.interior(lam).normal_()
+    other = torch.zeros_like(U0)
+    for _ in range(5):
+        grid.adjoint_step(U0, lam, other)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        grid.adjoint_step(U0, lam, other)
+        lam, other = other, lam
+    e1.record()
+    torch.cuda.synchronize()
+    adj_ms = e0.elapsed_time(e1) / reps
+    grid.take_flags(dev)
+    if log:
+        log(f"adjoint {n}^2: forward step {fwd_ms:.4f} ms, adjoint step {adj_ms:.4f} ms")
+
+    t_grad = t_fwd = 0.0
+    losses = []
+    flags = 0
+    import time
+    for s in range(solves):
+        W = workloads.adjoint_initial(n, seed=s, sigma=sigma)
+        g = workload_grid(W, n)
+        Wt = W.copy()
+        Wt[0] = Wt[0] + workloads.adjoint_perturbation(n, seed=s, sigma=sigma)
+        rho_t = g.interior(g.run(conserved_from_primitive(Wt, g), steps))[0] / g.volume
+        U = conserved_from_primitive(W, g)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.run(U, steps)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        U.requires_grad_(True)
+        loss = density_loss(rollout(U, g, steps), rho_t, g)
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_fwd += t1 - t0
+        t_grad += t2 - t1
+        losses.append(loss.item())
+        flags |= g.last_flags
+        assert torch.isfinite(U.grad).all()
+    return {"cells": f"{n}x{n}", "steps": steps, "limiter": "minmod", "riemann_solver": "hllc",
+            "forward_ms_per_step": fwd_ms, "adjoint_ms_per_step": adj_ms, "adjoint_to_forward": adj_ms / fwd_ms,
+            "solves": solves, "gradient_s_per_solve": t_grad / solves, "forward_run_s": t_fwd / solves,
+            "gradient_to_forward_run": t_grad / t_fwd, "vacuum_faces_flagged": bool(flags & VACUUM),
+            "checkpoint_every": int(math.ceil(math.sqrt(steps))), "mean_loss": sum(losses) / len(losses)}
