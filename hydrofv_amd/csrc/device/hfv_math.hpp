@@ -330,6 +330,30 @@ HFV_HD __forceinline__ void hllc_begin(double gamma, double tol, const double* w
     F.ps = g;
 }
 
+// After the first two powers of a two-rarefaction face (pzl = pow(pl, h/gamma),
+// pzr = pow(pr, h/gamma)): the guess is p* = pow(X, gamma/h), clamped to tol,
+// and HLLC uses it only through q_K (hllc_q), which is 1 whenever
+// p* <= p_K.  When X >= 0 lies below min(pzl, pzr) by a relative 2^-40 and
+// min(pl, pr) >= tol, p* (after the clamp) is below both pressures: the
+// margin covers the last-place errors of the powers and of X, and
+// (h/gamma)(gamma/h) != 1 in rounding, by orders of magnitude.  Then the
+// last power cannot change the flux, the face is settled with ps =
+// min(pl, pr) (q_L = q_R = 1, as with the reference's p*) and true is
+// returned; otherwise *X receives the base of the last power.
+HFV_HD __forceinline__ bool trrs_settle(double tol, HllcFace& F, double pzl, double pzr, double* X)
+{
+    const double x = trrs_base(F.ps, F.al, F.ar, pzl, pzr);
+    const double pz_lo = (pzl < pzr) ? pzl : pzr;
+    const double p_lo = (F.wl[3] > F.wr[3]) ? F.wr[3] : F.wl[3];
+    if (x >= 0.0 && x < pz_lo * (1.0 - 0x1p-40) && p_lo >= tol) {
+        F.ps = p_lo;
+        F.kind = HLLC_READY;
+        return true;
+    }
+    *X = x;
+    return false;
+}
+
 HFV_HD __forceinline__ int hllc_end(double gamma, const HllcFace& F, double* f)
 {
     if (F.kind == HLLC_VACUUM) {

@@ -131,8 +131,10 @@ __device__ __forceinline__ int yflux(double gamma, double tol, const double* hi,
 // lane (riemann_solver.c:191-199), evaluated 64 at a time over the
 // wavefront: the lanes' pending bases are packed into LDS, each lane raises
 // one packed value per pass, and the results are read back.  A row of a
-// strip has about 30 such faces in 128, so this takes two or three pow()
-// sequences per row instead of six.  q: 384 doubles of LDS; the workgroup is
+// strip has about 30 such faces in 128, so the first two powers take one or
+// two passes; the last power is needed only by the rare faces trrs_settle
+// cannot settle (about 1 in 1000 on a Kelvin-Helmholtz run), and its pass
+// is skipped when no lane has one.  q: 384 doubles of LDS; the workgroup is
 // this one wavefront, and every lane must call this.
 __device__ __forceinline__ void trrs_powers(double gamma, double tol, HllcFace& A, HllcFace& B, double* q, int lane)
 {
@@ -144,7 +146,7 @@ __device__ __forceinline__ void trrs_powers(double gamma, double tol, HllcFace& 
     const int ia = __popcll(ma & below), ib = na + __popcll(mb & below);
     const double h = 0.5 * (gamma - 1.0);
 
-    // stage 1: pow(pl, h/gamma) and pow(pr, h/gamma) at q[2i], q[2i+1]
+    // pow(pl, h/gamma) and pow(pr, h/gamma) of face i at q[2i], q[2i+1]
     if (ta) { q[2 * ia] = A.wl[3]; q[2 * ia + 1] = A.wr[3]; }
     if (tb) { q[2 * ib] = B.wl[3]; q[2 * ib + 1] = B.wr[3]; }
     __syncthreads();
@@ -154,24 +156,31 @@ __device__ __forceinline__ void trrs_powers(double gamma, double tol, HllcFace& 
         if (i < 2 * n) q[i] = pow(q[i], e1);
     }
     __syncthreads();
-    // stage 2: pow(base, gamma/h) at q[256 + i]
+    double xa = 0.0, xb = 0.0;
+    const bool la = ta && !trrs_settle(tol, A, q[2 * ia], q[2 * ia + 1], &xa);
+    const bool lb = tb && !trrs_settle(tol, B, q[2 * ib], q[2 * ib + 1], &xb);
+    const unsigned long long ra = __ballot(la), rb = __ballot(lb);
+    if ((ra | rb) == 0ull) return;
+    // pow(X, gamma/h) of the unsettled faces at q[256 + j]
+    const int nra = __popcll(ra), m = nra + __popcll(rb);
+    const int ja = __popcll(ra & below), jb = nra + __popcll(rb & below);
     double* q2 = q + 4 * LANES;
-    if (ta) q2[ia] = trrs_base(A.ps, A.al, A.ar, q[2 * ia], q[2 * ia + 1]);
-    if (tb) q2[ib] = trrs_base(B.ps, B.al, B.ar, q[2 * ib], q[2 * ib + 1]);
+    if (la) q2[ja] = xa;
+    if (lb) q2[jb] = xb;
     __syncthreads();
     const double e2 = gamma / h;
-    for (int b = 0; b < n; b += LANES) {
+    for (int b = 0; b < m; b += LANES) {
         const int i = b + lane;
-        if (i < n) q2[i] = pow(q2[i], e2);
+        if (i < m) q2[i] = pow(q2[i], e2);
     }
     __syncthreads();
-    if (ta) {
-        A.ps = q2[ia];
+    if (la) {
+        A.ps = q2[ja];
         if (A.ps < tol) A.ps = tol;
         A.kind = HLLC_READY;
     }
-    if (tb) {
-        B.ps = q2[ib];
+    if (lb) {
+        B.ps = q2[jb];
         if (B.ps < tol) B.ps = tol;
         B.kind = HLLC_READY;
     }

@@ -33,7 +33,7 @@ void hc_flux_hllc(int nf, double gamma, double tol, long long n, const double* w
 }
 
 // The 2D kernels' split form: hllc_begin, the two-rarefaction powers taken
-// as the wavefront takes them, hllc_end.
+// as the wavefront takes them (with trrs_settle), hllc_end.
 void hc_flux_hllc_split(double gamma, double tol, long long n, const double* wl, const double* wr, double* f,
                         int* err)
 {
@@ -43,9 +43,12 @@ void hc_flux_hllc_split(double gamma, double tol, long long n, const double* wl,
         if (F.kind == HLLC_TRRS) {
             const double h = 0.5 * (gamma - 1.0);
             const double pl = pow(F.wl[3], h / gamma), pr = pow(F.wr[3], h / gamma);
-            F.ps = pow(trrs_base(F.ps, F.al, F.ar, pl, pr), gamma / h);
-            if (F.ps < tol) F.ps = tol;
-            F.kind = HLLC_READY;
+            double x;
+            if (!trrs_settle(tol, F, pl, pr, &x)) {
+                F.ps = pow(x, gamma / h);
+                if (F.ps < tol) F.ps = tol;
+                F.kind = HLLC_READY;
+            }
         }
         err[i] = hllc_end(gamma, F, f + 4 * i);
     }

@@ -82,6 +82,40 @@ def test_split_hllc_equals_plain(hc):
     assert np.array_equal(f1[ok], f2[ok], equal_nan=True)
 
 
+def test_split_hllc_settles_exactly(hc):
+    """Two-rarefaction faces on both sides of trrs_settle's test: those it
+    settles without the last power and those it does not give the plain
+    flux bit for bit (and the oracle's)."""
+    r = np.random.default_rng(14)
+    n = 40000
+    wl = rng_states(n, 15, p=(1e-3, 50.0))
+    wr = rng_states(n, 16, p=(1e-3, 50.0))
+    wl[:, 1] = -r.uniform(0.0, 3.0, n)     # expansions: the two-rarefaction guess
+    wr[:, 1] = r.uniform(0.0, 3.0, n)
+    f1, f2 = np.empty((n, 4)), np.empty((n, 4))
+    e1, e2 = np.empty(n, np.int32), np.empty(n, np.int32)
+    hc.hc_flux_hllc(4, GAMMA, TOL, n, wl, wr, f1, e1)
+    hc.hc_flux_hllc_split(GAMMA, TOL, n, wl, wr, f2, e2)
+    assert np.array_equal(e1, e2)
+    ok = e1 == 0
+    assert np.array_equal(f1[ok], f2[ok], equal_nan=True)   # NaN: vacuum generation into 0/0
+    fo, eo = O.flux_batch(2, "riemann_solver_hllc", GAMMA, TOL, wl, wr)
+    assert np.array_equal(fo[ok], f1[ok], equal_nan=True)
+    # both outcomes occur among the two-rarefaction faces
+    al = np.sqrt(GAMMA * wl[:, 3] / wl[:, 0])
+    ar = np.sqrt(GAMMA * wr[:, 3] / wr[:, 0])
+    plo = np.minimum(wl[:, 3], wr[:, 3])
+    phi = np.maximum(wl[:, 3], wr[:, 3])
+    ppv = 0.5 * (wl[:, 3] + wr[:, 3]) - 0.125 * (wr[:, 1] - wl[:, 1]) * (wl[:, 0] + wr[:, 0]) * (al + ar)
+    trrs = ~((phi / plo <= 2) & (plo <= ppv) & (ppv <= phi)) & (ppv < plo) & ok
+    h = 0.5 * (GAMMA - 1.0)
+    with np.errstate(invalid="ignore"):
+        x = (al + ar - h * (wr[:, 1] - wl[:, 1])) / (al / wl[:, 3] ** (h / GAMMA) + ar / wr[:, 3] ** (h / GAMMA))
+        guess = x ** (GAMMA / h)
+    above = trrs & (guess > plo)
+    assert trrs.sum() > 10000 and above.sum() > 100, (trrs.sum(), above.sum())
+
+
 @pytest.mark.parametrize("nf", [3, 4])
 def test_host_exact_equals_oracle(hc, nf):
     wl, wr = K.riemann_states(3000, 13)
