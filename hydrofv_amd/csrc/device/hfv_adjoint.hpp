@@ -48,12 +48,12 @@ HFV_HD __forceinline__ int hllc_vjp(double gamma, double tol, const double* wl, 
         gb = 1;
         h = 0.5 * (gamma - 1.0);
         z = h / gamma;
-        pzl = pow(pl, z);
-        pzr = pow(pr, z);
+        pzl = pow_pos(pl, z);
+        pzr = pow_pos(pr, z);
         tnum = al + ar - h * (ur - ul);
         tden = al / pzl + ar / pzr;
         X = tnum / tden;
-        ps = pow(X, gamma / h);
+        ps = pow_pos(X, gamma / h);
     } else {
         gb = 2;
         Al = coeff_A(gamma, rl);

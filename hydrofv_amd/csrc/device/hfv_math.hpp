@@ -5,9 +5,10 @@
 // Every expression keeps the operand order of the reference C code it cites,
 // and the library is compiled with -ffp-contract=off, so each result is the
 // correctly rounded value of the same sequence of IEEE operations (divisions
-// and square roots are correctly rounded on this target).  pow() is the only
-// routine whose last-place rounding may differ from glibc's; it appears only
-// in the two-rarefaction guess, the exact solver and the vacuum samplers.
+// and square roots are correctly rounded on this target).  Powers are the
+// only results whose last place may differ from glibc's: those of the HLLC
+// two-rarefaction guess come from pow_pos (hfv_pow.hpp), the exact solver's
+// and the vacuum samplers' from the library pow.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,6 +16,8 @@
 #include <stdint.h>
 
 #define HFV_HD __host__ __device__
+
+#include "hfv_pow.hpp"
 
 namespace hfv {
 
@@ -99,14 +102,23 @@ HFV_HD __forceinline__ double trrs_base(double num, double al, double ar, double
     return num / (al / (pow_l) + ar / (pow_r));
 }
 
+// FAST_POW: the powers of positive pressures by pow_pos (hfv_pow.hpp), as
+// every HLLC evaluation of the kernels takes them; the exact solver's
+// initial guess keeps the library pow.
+template <bool FAST_POW>
 HFV_HD __forceinline__ double guess_p(double gamma, double rl, double ul, double pl, double al,
-                                          double rr, double ur, double pr, double ar, double tol)
+                                      double rr, double ur, double pr, double ar, double tol)
 {
     double guess;
     if (guess_select(gamma, rl, ul, pl, al, rr, ur, pr, ar, &guess)) {
         const double h = 0.5 * (gamma - 1.0);
-        guess = trrs_base(guess, al, ar, pow(pl, h / gamma), pow(pr, h / gamma));
-        guess = pow(guess, gamma / h);
+        if constexpr (FAST_POW) {
+            guess = trrs_base(guess, al, ar, pow_pos(pl, h / gamma), pow_pos(pr, h / gamma));
+            guess = pow_pos(guess, gamma / h);
+        } else {
+            guess = trrs_base(guess, al, ar, pow(pl, h / gamma), pow(pr, h / gamma));
+            guess = pow(guess, gamma / h);
+        }
     }
     if (guess < tol) guess = tol;
     return guess;
@@ -284,7 +296,7 @@ HFV_HD __forceinline__ int flux_hllc(double gamma, double tol, const double* wl,
         hllc_vacuum_flux<NF>(gamma, wl, al, wr, ar, f);
         return ERR_NONE;
     }
-    const double ps = guess_p(gamma, rl, ul, pl, al, rr, ur, pr, ar, tol);
+    const double ps = guess_p<true>(gamma, rl, ul, pl, al, rr, ur, pr, ar, tol);
     return hllc_from_guess<NF>(gamma, wl, wr, al, ar, ps, f);
 }
 
@@ -400,7 +412,7 @@ struct PStar {
 
 HFV_HD inline __noinline__ PStar solve_p_star(double gamma, Side L, Side R, double tol)
 {
-    const double pg = guess_p(gamma, L.r, L.u, L.p, L.a, R.r, R.u, R.p, R.a, tol);
+    const double pg = guess_p<false>(gamma, L.r, L.u, L.p, L.a, R.r, R.u, R.p, R.a, tol);
     bool bracketed = false;
     double p_hi = pg;
     double f_hi = pressure_fn(gamma, L, R, p_hi);

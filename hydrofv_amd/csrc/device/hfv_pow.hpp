@@ -1,0 +1,142 @@
+// hfv_pow.hpp -- pow(x, y) for the pressures of the two-rarefaction guess:
+// positive finite normal x, finite y, results well inside the double range.
+// Other arguments fall back to the library pow.
+//
+// Method (standard table-based log and exp in double-double arithmetic):
+//   x = 2^k m, m in [0.75, 1.5); c = RN(1/m_i) for m's interval i of width
+//   1/64; r = m c - 1 is carried exactly as a double-double (m c by a
+//   two-product, then - 1 exactly, since m c lies in [0.5, 2]);
+//   log x = k ln2 + (-log c) + log1p(r), |r| < 2^-6.5, log1p by its Taylor
+//   series to r^9, all sums as double-doubles: absolute error ~2^-66.
+//   w = y log x as a double-double; w = n ln2/64 + r (Cody-Waite, n*ln2/64
+//   split so n*LN2_64_HI is exact), |r| <= ln2/128; exp(r) - 1 by its
+//   Taylor series to r^6; 2^(n/64) = 2^(n>>6) * 2^(j/64) with 2^(j/64) a
+//   double-double table entry.
+// The result before its last rounding is within ~2^-64 relative of x^y, so
+// it is the correctly rounded x^y except when x^y lies within ~2^-11 ulp of
+// a rounding boundary.  Tables: hfv_pow_tables.hpp (gen_pow_tables.py).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "hfv_pow_tables.hpp"
+
+#ifndef HFV_HD
+#define HFV_HD __host__ __device__
+#endif
+
+namespace hfv {
+
+struct DD {
+    double hi, lo;
+};
+
+// Knuth's two-sum: a + b = s + e exactly
+HFV_HD __forceinline__ DD two_sum(double a, double b)
+{
+    const double s = a + b;
+    const double bb = s - a;
+    const double e = (a - (s - bb)) + (b - bb);
+    return DD{s, e};
+}
+
+// a + b = s + e exactly when |a| >= |b| (or a == 0)
+HFV_HD __forceinline__ DD fast_two_sum(double a, double b)
+{
+    const double s = a + b;
+    return DD{s, b - (s - a)};
+}
+
+HFV_HD __forceinline__ const double* pow_log_tab()
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return POW_LOG_TAB_D;
+#else
+    return POW_LOG_TAB_H;
+#endif
+}
+
+HFV_HD __forceinline__ const double* pow_exp_tab()
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return POW_EXP_TAB_D;
+#else
+    return POW_EXP_TAB_H;
+#endif
+}
+
+// log x as a double-double, x positive finite normal
+HFV_HD __forceinline__ DD log_dd(double x)
+{
+    int k;
+    double m = frexp(x, &k);  // m in [0.5, 1)
+    if (m < 0.75) {
+        m *= 2.0;
+        k -= 1;
+    }
+    const int i = (int)(m * 64.0) - 48;  // m in [0.75 + i/64, 0.75 + (i+1)/64)
+    const double* t = pow_log_tab() + 3 * i;
+    const double c = t[0];
+    // r = m c - 1 exactly: ph + pl = m c, ph - 1 exact (ph in [0.5, 2])
+    const double ph = m * c;
+    const double pl = fma(m, c, -ph);
+    const double rh = ph - 1.0;
+    // log1p(rh + pl) = rh + [pl (1 - rh) - rh^2 (1/2 - rh/3 + ... - rh^7/9)]
+    double q = -1.0 / 9.0;
+    q = fma(q, rh, 1.0 / 8.0);
+    q = fma(q, rh, -1.0 / 7.0);
+    q = fma(q, rh, 1.0 / 6.0);
+    q = fma(q, rh, -1.0 / 5.0);
+    q = fma(q, rh, 1.0 / 4.0);
+    q = fma(q, rh, -1.0 / 3.0);
+    q = fma(q, rh, 1.0 / 2.0);
+    const double tail = fma(pl, -rh, pl) - (rh * rh) * q;
+    // k ln2 + (-log c) + rh + (low parts)
+    const double kd = (double)k;
+    const DD s1 = two_sum(kd * POW_LN2_HI, t[1]);
+    const DD s2 = two_sum(s1.hi, rh);
+    const double low = ((s1.lo + s2.lo) + (kd * POW_LN2_LO + t[2])) + tail;
+    return fast_two_sum(s2.hi, low);
+}
+
+// exp(w.hi + w.lo) for |w.hi| < 700
+HFV_HD __forceinline__ double exp_dd(DD w)
+{
+    const double nd = rint(w.hi * POW_64_OVER_LN2);
+    const int n = (int)nd;
+    const int j = n & 63;
+    const int e = n >> 6;  // floor(n / 64)
+    // r = w - n ln2/64 as a double-double: the high difference is exact, and
+    // n*LN2_64_LO is not small next to it (up to ~2^-30), so renormalise
+    const DD r = two_sum(fma(-nd, POW_LN2_64_HI, w.hi), fma(-nd, POW_LN2_64_LO, w.lo));
+    const double rh = r.hi, rl = r.lo;
+    // exp(rh + rl) - 1 = rh + [rl (1 + rh) + rh^2 (1/2 + rh/6 + ... + rh^4/720)]
+    double q = 1.0 / 720.0;
+    q = fma(q, rh, 1.0 / 120.0);
+    q = fma(q, rh, 1.0 / 24.0);
+    q = fma(q, rh, 1.0 / 6.0);
+    q = fma(q, rh, 1.0 / 2.0);
+    const double b = fma(rl, rh, rl) + (rh * rh) * q;
+    const double* t = pow_exp_tab() + 2 * j;
+    const double th = t[0], tl = t[1];
+    // 2^(j/64) (1 + rh + b) = th + th rh + (th b + tl (1 + rh + b))
+    const double p = th * rh;
+    const double pe = fma(th, rh, -p);
+    const DD s = fast_two_sum(th, p);
+    const double low = ((s.lo + pe) + fma(th, b, tl * (1.0 + (rh + b))));
+    return ldexp(s.hi + low, e);
+}
+
+// x^y; the fast path covers positive finite normal x with |y log x| < 700
+HFV_HD __forceinline__ double pow_pos(double x, double y)
+{
+    if (!(x >= 0x1p-1022 && x <= 0x1.fffffffffffffp+1023)) return pow(x, y);
+    const DD l = log_dd(x);
+    const double wh = y * l.hi;
+    if (!(fabs(wh) < 700.0)) return pow(x, y);
+    const double wl = fma(y, l.hi, -wh) + y * l.lo;
+    return exp_dd(DD{wh, wl});
+}
+
+}  // namespace hfv

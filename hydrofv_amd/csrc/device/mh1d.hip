@@ -227,6 +227,13 @@ __global__ void k_probe_flux4(int solver, double gamma, double tol, long long n,
     err[i] = e;
 }
 
+constexpr int PROBE_SPLIT_MISMATCH = 99;
+
+__device__ __forceinline__ bool same_bits(double a, double b)
+{
+    return __double_as_longlong(a) == __double_as_longlong(b) || (isnan(a) && isnan(b));
+}
+
 // The 2D kernels' HLLC form: hllc_begin, the wavefront's two-rarefaction
 // powers, hllc_end.  Lane l of a one-wavefront block takes state pairs i and
 // n + i (when they exist), so every lane holds two faces like a strip row.
@@ -250,8 +257,18 @@ __global__ void __launch_bounds__(LANES) k_probe_flux4_wave(double gamma, double
     hllc_begin(gamma, tol, lb, rb, B);
     trrs_powers(gamma, tol, A, B, q, lane);
     double oa[4], ob[4];
-    const int ea = hllc_end(gamma, A, oa);
-    const int eb = hllc_end(gamma, B, ob);
+    int ea = hllc_end(gamma, A, oa);
+    int eb = hllc_end(gamma, B, ob);
+    // the split form must give the plain flux_hllc<4> bits (the kernels use
+    // both for the same faces, depending on the strip layout)
+    double pa[4], pb[4];
+    const int fa = flux_hllc<4>(gamma, tol, la, ra, pa);
+    const int fb = flux_hllc<4>(gamma, tol, lb, rb, pb);
+    bool sa = fa == ea, sb = fb == eb;
+    for (int k = 0; k < 4 && fa == 0 && ea == 0; k++) sa = sa && same_bits(pa[k], oa[k]);
+    for (int k = 0; k < 4 && fb == 0 && eb == 0; k++) sb = sb && same_bits(pb[k], ob[k]);
+    if (!sa) ea = PROBE_SPLIT_MISMATCH;
+    if (!sb) eb = PROBE_SPLIT_MISMATCH;
     if (i < half) {
         for (int k = 0; k < 4; k++) f[4 * ia + k] = oa[k];
         err[ia] = ea;
@@ -260,6 +277,13 @@ __global__ void __launch_bounds__(LANES) k_probe_flux4_wave(double gamma, double
         for (int k = 0; k < 4; k++) f[4 * ib + k] = ob[k];
         err[ib] = eb;
     }
+}
+
+__global__ void k_probe_pow(int impl, long long n, const double* x, const double* y, double* out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = impl ? pow_pos(x[i], y[i]) : pow(x[i], y[i]);
 }
 
 __global__ void k_probe_limit(int lim, long long n, const double* a, const double* b, double* out)
@@ -381,6 +405,21 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
     st->smax[0] = smax_from_key(c.key[0]);
     st->smax[1] = 0.0;
     st->finished = 1;
+    return HFV_OK;
+}
+
+int hfv_probe_pow(int impl, long long n, const double* x, const double* y, double* out)
+{
+    if (n < 0 || (impl != 0 && impl != 1)) return HFV_EINVAL;
+    if (n == 0) return HFV_OK;
+    DevBuf<double> dx, dy, dout;
+    if (!dx.alloc((size_t)n) || !dy.alloc((size_t)n) || !dout.alloc((size_t)n)) return HFV_ENOMEM;
+    if (hipMemcpy(dx.p, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dy.p, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return HFV_EDEVICE;
+    hipLaunchKernelGGL(k_probe_pow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, impl, n, dx.p, dy.p, dout.p);
+    if (hipGetLastError() != hipSuccess) return HFV_EDEVICE;
+    if (hipMemcpy(out, dout.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return HFV_EDEVICE;
     return HFV_OK;
 }
 

@@ -26,6 +26,7 @@
 
 #include "hfv_ctrl.hpp"
 #include "hfv_math.hpp"
+#include "hfv_pow.hpp"
 
 namespace hfv {
 
@@ -130,7 +131,8 @@ __device__ __forceinline__ int yflux(double gamma, double tol, const double* hi,
 // The powers of the two-rarefaction pressure guesses pending on two faces per
 // lane (riemann_solver.c:191-199), evaluated 64 at a time over the
 // wavefront: the lanes' pending bases are packed into LDS, each lane raises
-// one packed value per pass, and the results are read back.  A row of a
+// one packed value per pass (pow_pos: the fast path of hfv_pow.hpp for
+// positive pressures), and the results are read back.  A row of a
 // strip has about 30 such faces in 128, so the first two powers take one or
 // two passes; the last power is needed only by the rare faces trrs_settle
 // cannot settle (about 1 in 1000 on a Kelvin-Helmholtz run), and its pass
@@ -153,7 +155,7 @@ __device__ __forceinline__ void trrs_powers(double gamma, double tol, HllcFace& 
     const double e1 = h / gamma;
     for (int b = 0; b < 2 * n; b += LANES) {
         const int i = b + lane;
-        if (i < 2 * n) q[i] = pow(q[i], e1);
+        if (i < 2 * n) q[i] = pow_pos(q[i], e1);
     }
     __syncthreads();
     double xa = 0.0, xb = 0.0;
@@ -171,7 +173,7 @@ __device__ __forceinline__ void trrs_powers(double gamma, double tol, HllcFace& 
     const double e2 = gamma / h;
     for (int b = 0; b < m; b += LANES) {
         const int i = b + lane;
-        if (i < m) q2[i] = pow(q2[i], e2);
+        if (i < m) q2[i] = pow_pos(q2[i], e2);
     }
     __syncthreads();
     if (la) {

@@ -75,6 +75,7 @@ PROTOTYPES = {
     "hfv_probe_flux": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_longlong, dp, dp, dp,
                                  _P(C.c_int)]),
     "hfv_probe_limit": (C.c_int, [C.c_int, C.c_longlong, dp, dp, dp]),
+    "hfv_probe_pow": (C.c_int, [C.c_int, C.c_longlong, dp, dp, dp]),
     "hfv_device_error_string": (C.c_char_p, []),
     "hfv_muscl_hancock_1d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
     "hfv_muscl_hancock_2d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
@@ -123,6 +124,15 @@ def probe_limit(lib, limiter: int, a: np.ndarray, b: np.ndarray):
     b = np.ascontiguousarray(b, dtype=np.float64)
     out = np.zeros_like(a)
     check(lib, lib.hfv_probe_limit(limiter, a.size, _ptr(a), _ptr(b), _ptr(out)), "hfv_probe_limit")
+    return out
+
+
+def probe_pow(lib, impl: int, x: np.ndarray, y: np.ndarray):
+    """x^y on the device: impl 0 the library pow, 1 the kernels' pow_pos."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros_like(x)
+    check(lib, lib.hfv_probe_pow(impl, x.size, _ptr(x), _ptr(y), _ptr(out)), "hfv_probe_pow")
     return out
 
 

@@ -149,16 +149,22 @@ enum {
 
 /* V = step(U); U and V must not overlap. */
 int hfv_fwd2d_step(const hfv_adj2d_config *cfg, double *const U[4], double *const V[4], int *flags, void *stream);
-/* lam_out = lam_in + (d step(U) / dU)^T lam_in on the interior (lam_in: the
- * cotangent of step(U), lam_out: that of U); lam_out must not overlap U or
+/* lam_out = (d step(U) / dU)^T lam_in on the interior (lam_in: the
+ * cotangent of step(U), lam_out: that of U; the step's identity part makes
+ * this lam_in plus the flux terms' transposed Jacobian); lam_out must not overlap U or
  * lam_in, and its ghost entries are left untouched. */
 int hfv_adj2d_step(const hfv_adj2d_config *cfg, double *const U[4], double *const lam_in[4],
                    double *const lam_out[4], int *flags, void *stream);
 
 /* ---- batched evaluation of the device physics (parity probes) ---------- */
-/* f[n][nf] = flux of (wl[n][nf], wr[n][nf]); nf 3 (1D) or 4 (2D); err[n]. */
+/* f[n][nf] = flux of (wl[n][nf], wr[n][nf]); nf 3 (1D) or 4 (2D); err[n].  The 2D HLLC
+ * probe evaluates the kernels' split form and checks it against the plain one:
+ * err 99 marks a state where the two disagree. */
 int hfv_probe_flux(int nf, int solver, double gamma, double tol, long long n, const double *wl,
                    const double *wr, double *f, int *err);
+/* out[n] = x[n]^y[n]: impl 0 the device library pow, impl 1 the kernels'
+ * pow for positive pressures (two-rarefaction guess, riemann_solver.c:191-199). */
+int hfv_probe_pow(int impl, long long n, const double *x, const double *y, double *out);
 /* out[n] = limit_slope(a[n], b[n]) for limiter 1/2/3 (slope_limiter.c:93-115). */
 int hfv_probe_limit(int limiter, long long n, const double *a, const double *b, double *out);
 

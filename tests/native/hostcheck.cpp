@@ -4,6 +4,7 @@
 // the oracle, derivative pieces against automatic differentiation.
 // Test infrastructure only; built by tests/native/Makefile.
 #include "../../hydrofv_amd/csrc/device/hfv_adjoint.hpp"
+#include "../../hydrofv_amd/csrc/device/hfv_pow.hpp"
 
 using namespace hfv;
 
@@ -21,6 +22,12 @@ void limits(long long n, const double* a, const double* b, double* d, double* da
 }  // namespace
 
 extern "C" {
+
+// out[n] = pow_pos(x[n], y[n]) (the kernels' power of positive pressures)
+void hc_pow_pos(long long n, const double* x, const double* y, double* out)
+{
+    for (long long i = 0; i < n; i++) out[i] = pow_pos(x[i], y[i]);
+}
 
 // f[n][nf] = HLLC flux (flux_hllc<nf>); err[n]
 void hc_flux_hllc(int nf, double gamma, double tol, long long n, const double* wl, const double* wr, double* f,
@@ -42,10 +49,10 @@ void hc_flux_hllc_split(double gamma, double tol, long long n, const double* wl,
         hllc_begin(gamma, tol, wl + 4 * i, wr + 4 * i, F);
         if (F.kind == HLLC_TRRS) {
             const double h = 0.5 * (gamma - 1.0);
-            const double pl = pow(F.wl[3], h / gamma), pr = pow(F.wr[3], h / gamma);
+            const double pl = pow_pos(F.wl[3], h / gamma), pr = pow_pos(F.wr[3], h / gamma);
             double x;
             if (!trrs_settle(tol, F, pl, pr, &x)) {
-                F.ps = pow(x, gamma / h);
+                F.ps = pow_pos(x, gamma / h);
                 if (F.ps < tol) F.ps = tol;
                 F.kind = HLLC_READY;
             }

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 import _cases as K
-from hydrofv_amd import driver
+from hydrofv_amd import driver, gpu
 
 pytestmark = pytest.mark.gpu
 
@@ -212,3 +212,36 @@ def test_hllc_invalid_wave_speed_error(gpulib):
         sim.launch("muscl_hancock_2d", "riemann_solver_hllc", "minmod", 0.5, tf=1.0)
     assert e.value.code == 2
     assert "Invalid wave speed for the HLLC riemann solver" in e.value.text
+
+
+def test_probe_pow_pos_matches_host_build(gpulib):
+    """The kernels' pow for positive pressures gives the same bits on the
+    device as its host build (tests/native), which test_adjoint_math checks
+    against glibc and against correctly rounded values; the device library
+    pow is reported beside it."""
+    import ctypes as C
+    import math
+    import os
+    r = np.random.default_rng(5)
+    n = 200_000
+    x = np.exp(r.uniform(np.log(1e-6), np.log(1e4), n))
+    gam = r.uniform(1.05, 3.0, n)
+    h = 0.5 * (gam - 1.0)
+    y = np.where(r.uniform(size=n) < 0.7, h / gam, gam / h)
+    keep = np.abs(y * np.log(x)) < 690
+    x, y = x[keep], y[keep]
+    dev = gpu.probe_pow(gpulib, 1, x, y)
+    lib_pow = gpu.probe_pow(gpulib, 0, x, y)
+    ref = np.array([math.pow(a, b) for a, b in zip(x.tolist(), y.tolist())])
+    so = os.path.join(os.path.dirname(os.path.abspath(__file__)), "native", "libhfv_hostcheck.so")
+    if os.path.exists(so):
+        L = C.CDLL(so)
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        L.hc_pow_pos.argtypes = [C.c_longlong, dp, dp, dp]
+        L.hc_pow_pos.restype = None
+        host = np.empty_like(x)
+        L.hc_pow_pos(len(x), x, y, host)
+        assert np.array_equal(dev, host)
+    assert np.all(np.abs(dev - ref) <= np.spacing(np.abs(ref)))
+    print(f"pow vs glibc over {len(x)} pressure-guess arguments: pow_pos differs in {np.mean(dev != ref):.2e}, "
+          f"device library pow in {np.mean(lib_pow != ref):.2e}")
