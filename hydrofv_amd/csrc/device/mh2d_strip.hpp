@@ -281,13 +281,22 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             }
             hllc_begin(gamma, tol, hl, xlo, Fx);
             trrs_powers(gamma, tol, Fy, Fx, trrs_q, lane);
+            // both faces' regular fluxes back to back, then the rare vacuum faces
             double o[4];
-            ey = hllc_end(gamma, Fy, o);
+            ey = hllc_end_regular(gamma, Fy, o);
+            ex = hllc_end_regular(gamma, Fx, fxl);
+            if (Fy.kind == HLLC_VACUUM) {
+                hllc_vacuum_flux<4>(gamma, Fy.wl, Fy.al, Fy.wr, Fy.ar, o);
+                ey = ERR_NONE;
+            }
+            if (Fx.kind == HLLC_VACUUM) {
+                hllc_vacuum_flux<4>(gamma, Fx.wl, Fx.al, Fx.wr, Fx.ar, fxl);
+                ex = ERR_NONE;
+            }
             fyt[0] = o[0];
             fyt[1] = o[2];
             fyt[2] = o[1];
             fyt[3] = o[3];
-            ex = hllc_end(gamma, Fx, fxl);
         } else {
             ey = yflux<SOLVER>(gamma, tol, yt, ylo, fyt);
             ex = flux<SOLVER, 4>(gamma, tol, hl, xlo, fxl);
