@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(LANES) k_fwd_step(Grid2D G, double dt, Fields4
 {
     const int ntiles = G.nsx * G.nsy;
     const int b = xcd_remap(blockIdx.x, ntiles);
-    const StripResult r = strip_step<LIM, HLLC, false>(G, dt, U, V, U, b % G.nsx, b / G.nsx, threadIdx.x);
+    const StripResult r = strip_step<LIM, HLLC, false, false>(G, dt, U, V, U, b % G.nsx, b / G.nsx, threadIdx.x);
     if (__ballot(r.err != ERR_NONE) && threadIdx.x == 0) atomicOr(flags, HFV_ADJ_FLUX_ERROR);
 }
 

@@ -29,8 +29,8 @@ struct EnsInstance {
     Ctrl* ctrl;  // 3 slots
 };
 
-template <int LIM, int SOLVER, bool WMODE>
-__global__ void __launch_bounds__(LANES) k_ens_step(const EnsInstance* __restrict__ insts, const int2* __restrict__ tiles,
+template <int LIM, int SOLVER, bool WMODE, bool BRANCHY>
+__global__ void __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(SOLVER == 2 ? 2 : 1))) k_ens_step(const EnsInstance* __restrict__ insts, const int2* __restrict__ tiles,
                                                     int ntiles, int slot)
 {
     const int b = xcd_remap(blockIdx.x, ntiles);
@@ -47,7 +47,8 @@ __global__ void __launch_bounds__(LANES) k_ens_step(const EnsInstance* __restric
     if (!plan.run) return;
     const Fields4 U = plan.cur ? E.Ub : E.Ua;
     const Fields4 V = plan.cur ? E.Ua : E.Ub;
-    const StripResult r = strip_step<LIM, SOLVER, WMODE>(G, plan.dt, U, V, E.W, t.y % G.nsx, t.y / G.nsx, lane);
+    const StripResult r = strip_step<LIM, SOLVER, WMODE, BRANCHY>(G, plan.dt, U, V, E.W, t.y % G.nsx, t.y / G.nsx,
+                                                                     lane);
     strip_finish(r, out, lane);
 }
 
@@ -96,7 +97,7 @@ using namespace hfv;
 
 struct hfv_ens {
     int count;
-    int limiter, solver;
+    int limiter, solver, form;
     std::vector<hfv_mh2d_config> cfg;
     std::vector<EnsInstance> host;  // descriptors (device pointers inside)
     EnsInstance* dinst;
@@ -147,29 +148,36 @@ int build_tiles(hfv_ens* e)
     return HFV_OK;
 }
 
-template <int LIM, int SOLVER>
+template <int LIM, int SOLVER, bool BRANCHY>
 void ens_launch_t(hfv_ens* e, bool wmode, hipStream_t s)
 {
     const int slot = (int)(e->launches % 3);
     if (wmode)
-        hipLaunchKernelGGL((k_ens_step<LIM, SOLVER, true>), dim3(e->ntiles), dim3(LANES), 0, s, e->dinst, e->dtiles,
-                           e->ntiles, slot);
+        hipLaunchKernelGGL((k_ens_step<LIM, SOLVER, true, BRANCHY>), dim3(e->ntiles), dim3(LANES), 0, s, e->dinst,
+                           e->dtiles, e->ntiles, slot);
     else
-        hipLaunchKernelGGL((k_ens_step<LIM, SOLVER, false>), dim3(e->ntiles), dim3(LANES), 0, s, e->dinst, e->dtiles,
-                           e->ntiles, slot);
+        hipLaunchKernelGGL((k_ens_step<LIM, SOLVER, false, BRANCHY>), dim3(e->ntiles), dim3(LANES), 0, s, e->dinst,
+                           e->dtiles, e->ntiles, slot);
     e->launches++;
+}
+
+template <int LIM>
+void ens_launch_hllc(hfv_ens* e, bool wmode, hipStream_t s)
+{
+    if (e->form == HFV_HLLC_BRANCHES) ens_launch_t<LIM, HLLC, true>(e, wmode, s);
+    else ens_launch_t<LIM, HLLC, false>(e, wmode, s);
 }
 
 void ens_launch(hfv_ens* e, bool wmode, hipStream_t s)
 {
     if (e->solver == HLLC) {
-        if (e->limiter == MINMOD) ens_launch_t<MINMOD, HLLC>(e, wmode, s);
-        else if (e->limiter == VAN_LEER) ens_launch_t<VAN_LEER, HLLC>(e, wmode, s);
-        else ens_launch_t<MC, HLLC>(e, wmode, s);
+        if (e->limiter == MINMOD) ens_launch_hllc<MINMOD>(e, wmode, s);
+        else if (e->limiter == VAN_LEER) ens_launch_hllc<VAN_LEER>(e, wmode, s);
+        else ens_launch_hllc<MC>(e, wmode, s);
     } else {
-        if (e->limiter == MINMOD) ens_launch_t<MINMOD, EXACT>(e, wmode, s);
-        else if (e->limiter == VAN_LEER) ens_launch_t<VAN_LEER, EXACT>(e, wmode, s);
-        else ens_launch_t<MC, EXACT>(e, wmode, s);
+        if (e->limiter == MINMOD) ens_launch_t<MINMOD, EXACT, false>(e, wmode, s);
+        else if (e->limiter == VAN_LEER) ens_launch_t<VAN_LEER, EXACT, false>(e, wmode, s);
+        else ens_launch_t<MC, EXACT, false>(e, wmode, s);
     }
 }
 
@@ -189,6 +197,7 @@ int hfv_ens_create(int count, const hfv_mh2d_config* cfgs, hfv_ens** out)
         if (c.slope_limiter != cfgs[0].slope_limiter || c.riemann_solver != cfgs[0].riemann_solver) return HFV_EINVAL;
         if (c.slope_limiter < 1 || c.slope_limiter > 3) return HFV_EINVAL;
         if (c.riemann_solver != HLLC && c.riemann_solver != EXACT) return HFV_EINVAL;
+        if (c.hllc_form != HFV_HLLC_SELECTS && c.hllc_form != HFV_HLLC_BRANCHES) return HFV_EINVAL;
         for (int s = 0; s < 4; s++)
             if (c.bc[s] < BC_NONE || c.bc[s] > BC_PERIODIC) return HFV_EINVAL;
     }
@@ -197,6 +206,7 @@ int hfv_ens_create(int count, const hfv_mh2d_config* cfgs, hfv_ens** out)
     e->count = count;
     e->limiter = cfgs[0].slope_limiter;
     e->solver = cfgs[0].riemann_solver;
+    e->form = cfgs[0].hllc_form;
     e->cfg.assign(cfgs, cfgs + count);
     e->host.resize(count);
     size_t total = 0;
@@ -297,6 +307,17 @@ int hfv_ens_set_strip_rows(hfv_ens* e, const int* rows)
         G.nsy = (G.ny + G.strip_rows - 1) / G.strip_rows;
     }
     return build_tiles(e);
+}
+
+int hfv_ens_set_hllc_form(hfv_ens* e, int form)
+{
+    if (form != HFV_HLLC_SELECTS && form != HFV_HLLC_BRANCHES) return HFV_EINVAL;
+    if (form != e->form) {
+        e->form = form;
+        if (e->graph) (void)hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    return HFV_OK;
 }
 
 int hfv_ens_num_tiles(hfv_ens* e) { return e->ntiles; }

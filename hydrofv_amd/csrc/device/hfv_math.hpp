@@ -102,8 +102,8 @@ HFV_HD __forceinline__ double trrs_base(double num, double al, double ar, double
     return num / (al / (pow_l) + ar / (pow_r));
 }
 
-// FAST_POW: the powers of positive pressures by pow_pos (hfv_pow.hpp), as
-// every HLLC evaluation of the kernels takes them; the exact solver's
+// FAST_POW: the powers by pow_pos (hfv_pow.hpp), as every HLLC evaluation
+// of the kernels takes them; the exact solver's
 // initial guess keeps the library pow.
 template <bool FAST_POW>
 HFV_HD __forceinline__ double guess_p(double gamma, double rl, double ul, double pl, double al,
@@ -315,84 +315,162 @@ struct HllcFace {
     int kind;
 };
 
-// hllc_begin and hllc_end_regular evaluate every candidate of a branch
-// (pressure-guess kind, q_K, wave region, star state) and select, so that
-// the compiler can interleave the independent evaluations of a lane's two
-// faces; each selected value comes from the same expression as in
-// flux_hllc, so the bits are the same.
-HFV_HD __forceinline__ void hllc_begin(double gamma, double tol, const double* wl, const double* wr, HllcFace& F)
+// hllc_begin_faces and hllc_end_regular_faces evaluate every candidate of a
+// branch (pressure-guess kind, q_K, wave region, star state) and select, for
+// N faces at once, so that the compiler can interleave the independent
+// evaluations of a lane's faces; each selected value comes from the same
+// expression as in flux_hllc, so the bits are the same.  F[i].wl and F[i].wr
+// are set by the caller.
+template <int N>
+HFV_HD __forceinline__ void hllc_begin_faces(double gamma, double tol, HllcFace (&F)[N])
 {
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        F.wl[k] = wl[k];
-        F.wr[k] = wr[k];
-    }
-    const double rl = wl[0], ul = wl[1], pl = wl[3];
-    const double rr = wr[0], ur = wr[1], pr = wr[3];
-    const double al_ = sound_speed(gamma, rl, pl), ar_ = sound_speed(gamma, rr, pr);
-    const double al = (rl > 0.0) ? al_ : 0.0;
-    const double ar = (rr > 0.0) ? ar_ : 0.0;
-    F.al = al;
-    F.ar = ar;
-    const bool vac = hllc_is_vacuum(gamma, rl, ul, al, rr, ur, ar);
-    // riemann_solver.c:154-224 (as guess_select)
-    const double plo = (pl > pr) ? pr : pl;
-    const double phi = (pl > pr) ? pl : pr;
-    const double ppv = 0.5 * (pl + pr) - 0.125 * (ur - ul) * (rl + rr) * (al + ar);
-    const bool pvrs = (phi / plo <= 2.0) & (plo <= ppv) & (ppv <= phi);
-    const bool trrs = !pvrs & (ppv < plo);
     const double h = 0.5 * (gamma - 1.0);
-    const double num = al + ar - h * (ur - ul);
-    const double gl = sqrt(coeff_A(gamma, rl) / (ppv + coeff_B(gamma, pl)));
-    const double gr = sqrt(coeff_A(gamma, rr) / (ppv + coeff_B(gamma, pr)));
-    const double tsrs = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
-    double g = pvrs ? ppv : tsrs;
-    g = (g < tol) ? tol : g;
-    F.ps = vac ? 0.0 : (trrs ? num : g);
-    F.kind = vac ? HLLC_VACUUM : (trrs ? HLLC_TRRS : HLLC_READY);
+    double ppv[N], num[N], tsrs[N];
+    bool vac[N], pvrs[N], trrs[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const double rl = F[i].wl[0], ul = F[i].wl[1], pl = F[i].wl[3];
+        const double rr = F[i].wr[0], ur = F[i].wr[1], pr = F[i].wr[3];
+        const double al_ = sound_speed(gamma, rl, pl), ar_ = sound_speed(gamma, rr, pr);
+        const double al = (rl > 0.0) ? al_ : 0.0;
+        const double ar = (rr > 0.0) ? ar_ : 0.0;
+        F[i].al = al;
+        F[i].ar = ar;
+        vac[i] = hllc_is_vacuum(gamma, rl, ul, al, rr, ur, ar);
+        // riemann_solver.c:154-224 (as guess_select)
+        const double plo = (pl > pr) ? pr : pl;
+        const double phi = (pl > pr) ? pl : pr;
+        ppv[i] = 0.5 * (pl + pr) - 0.125 * (ur - ul) * (rl + rr) * (al + ar);
+        pvrs[i] = (phi / plo <= 2.0) & (plo <= ppv[i]) & (ppv[i] <= phi);
+        trrs[i] = !pvrs[i] & (ppv[i] < plo);
+        num[i] = al + ar - h * (ur - ul);
+    }
+    // the two-shock candidates as a phase of their own
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const double rl = F[i].wl[0], ul = F[i].wl[1], pl = F[i].wl[3];
+        const double rr = F[i].wr[0], ur = F[i].wr[1], pr = F[i].wr[3];
+        const double gl = sqrt(coeff_A(gamma, rl) / (ppv[i] + coeff_B(gamma, pl)));
+        const double gr = sqrt(coeff_A(gamma, rr) / (ppv[i] + coeff_B(gamma, pr)));
+        tsrs[i] = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double g = pvrs[i] ? ppv[i] : tsrs[i];
+        g = (g < tol) ? tol : g;
+        F[i].ps = vac[i] ? 0.0 : (trrs[i] ? num[i] : g);
+        F[i].kind = vac[i] ? HLLC_VACUUM : (trrs[i] ? HLLC_TRRS : HLLC_READY);
+    }
 }
 
-// The flux of hllc_from_guess<4> from a ready face, whatever its kind (a
-// vacuum face's result is replaced by the caller): hllc.c:68-371.
-HFV_HD __forceinline__ int hllc_end_regular(double gamma, const HllcFace& F, double* f)
+HFV_HD __forceinline__ void hllc_begin(double gamma, double tol, const double* wl, const double* wr, HllcFace& F)
 {
-    const double rl = F.wl[0], ul = F.wl[1], vl = F.wl[2], pl = F.wl[3];
-    const double rr = F.wr[0], ur = F.wr[1], vr = F.wr[2], pr = F.wr[3];
-    const double al = F.al, ar = F.ar, ps = F.ps;
+    HllcFace G[1];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        G[0].wl[k] = wl[k];
+        G[0].wr[k] = wr[k];
+    }
+    hllc_begin_faces<1>(gamma, tol, G);
+    F = G[0];
+}
+
+// The fluxes of hllc_from_guess<4> from ready faces, whatever their kind (a
+// vacuum face's result is replaced by the caller): hllc.c:68-371.
+template <int N>
+HFV_HD __forceinline__ void hllc_end_regular_faces(double gamma, const HllcFace (&F)[N], double (&f)[N][4],
+                                                   int (&err)[N])
+{
     // hllc_q (hllc.c:30-40) on both sides
     const double kq = 0.5 + 0.5 / gamma;
-    const double qls = sqrt(1.0 + kq * (ps / pl - 1.0));
-    const double qrs = sqrt(1.0 + kq * (ps / pr - 1.0));
-    const double ql = (ps <= pl) ? 1.0 : qls;
-    const double qr = (ps <= pr) ? 1.0 : qrs;
-    const double sl = ul - al * ql;
-    const double sr = ur + ar * qr;
-    const double sm = (pr - pl + rl * ul * (sl - ul) - rr * ur * (sr - ur)) / (rl * (sl - ul) - rr * (sr - ur));
-    // the regions of hllc.c:302-371, tested in the same order
-    const bool r0 = 0.0 <= sl;
-    const bool r1 = !r0 & (sl <= 0.0) & (0.0 <= sm);
-    const bool r2 = !r0 & !r1 & (sm <= 0.0) & (0.0 <= sr);
-    const bool r3 = !r0 & !r1 & !r2 & (sr <= 0.0);
-    const bool left = r0 | r1;
-    const bool star = r1 | r2;
-    const double r = left ? rl : rr;
-    const double u = left ? ul : ur;
-    const double v = left ? vl : vr;
-    const double p = left ? pl : pr;
-    const double s = left ? sl : sr;
-    const double e = r * (0.5 * (u * u + v * v) + p / (r * (gamma - 1.0)));
-    const double fm = r * u;
-    const double fu = fm * u + p;
-    const double fe = u * (e + p);
-    const double rs = r * (s - u) / (s - sm);
-    const double mus = rs * sm;
-    const double es = rs * (e / r + (sm - u) * (sm + p / (r * (s - u))));
-    const double mvs = rs * v;
-    f[0] = star ? fm + s * (rs - r) : fm;
-    f[1] = star ? fu + s * (mus - r * u) : fu;
-    f[2] = star ? fm * v + s * (mvs - r * v) : fm * v;
-    f[3] = star ? fe + s * (es - e) : fe;
-    return (r0 | r1 | r2 | r3) ? ERR_NONE : ERR_HLLC_SPEED;
+    double ql[N], qr[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const double ps = F[i].ps, pl = F[i].wl[3], pr = F[i].wr[3];
+        const double qls = sqrt(1.0 + kq * (ps / pl - 1.0));
+        const double qrs = sqrt(1.0 + kq * (ps / pr - 1.0));
+        ql[i] = (ps <= pl) ? 1.0 : qls;
+        qr[i] = (ps <= pr) ? 1.0 : qrs;
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const double rl = F[i].wl[0], ul = F[i].wl[1], vl = F[i].wl[2], pl = F[i].wl[3];
+        const double rr = F[i].wr[0], ur = F[i].wr[1], vr = F[i].wr[2], pr = F[i].wr[3];
+        const double al = F[i].al, ar = F[i].ar;
+        const double sl = ul - al * ql[i];
+        const double sr = ur + ar * qr[i];
+        const double sm = (pr - pl + rl * ul * (sl - ul) - rr * ur * (sr - ur)) / (rl * (sl - ul) - rr * (sr - ur));
+        // the regions of hllc.c:302-371, tested in the same order
+        const bool r0 = 0.0 <= sl;
+        const bool r1 = !r0 & (sl <= 0.0) & (0.0 <= sm);
+        const bool r2 = !r0 & !r1 & (sm <= 0.0) & (0.0 <= sr);
+        const bool r3 = !r0 & !r1 & !r2 & (sr <= 0.0);
+        const bool left = r0 | r1;
+        const bool star = r1 | r2;
+        const double r = left ? rl : rr;
+        const double u = left ? ul : ur;
+        const double v = left ? vl : vr;
+        const double p = left ? pl : pr;
+        const double s = left ? sl : sr;
+        const double e = r * (0.5 * (u * u + v * v) + p / (r * (gamma - 1.0)));
+        const double fm = r * u;
+        const double fu = fm * u + p;
+        const double fe = u * (e + p);
+        const double rs = r * (s - u) / (s - sm);
+        const double mus = rs * sm;
+        const double es = rs * (e / r + (sm - u) * (sm + p / (r * (s - u))));
+        const double mvs = rs * v;
+        f[i][0] = star ? fm + s * (rs - r) : fm;
+        f[i][1] = star ? fu + s * (mus - r * u) : fu;
+        f[i][2] = star ? fm * v + s * (mvs - r * v) : fm * v;
+        f[i][3] = star ? fe + s * (es - e) : fe;
+        err[i] = (r0 | r1 | r2 | r3) ? ERR_NONE : ERR_HLLC_SPEED;
+    }
+}
+
+HFV_HD __forceinline__ int hllc_end_regular(double gamma, const HllcFace& F, double* f)
+{
+    const HllcFace G[1] = {F};
+    double o[1][4];
+    int e[1];
+    hllc_end_regular_faces<1>(gamma, G, o, e);
+#pragma unroll
+    for (int k = 0; k < 4; k++) f[k] = o[0][k];
+    return e[0];
+}
+
+// The same face set-up and flux with branches in place of the selects
+// (hllc.c:68-371 and riemann_solver.c:154-224 as written): cheaper than the
+// selects where the lanes of a wavefront agree on the branches, as in
+// uniform regions; the bits are the same.
+HFV_HD __forceinline__ void hllc_begin_branchy(double gamma, double tol, HllcFace& F)
+{
+    const double rl = F.wl[0], ul = F.wl[1], pl = F.wl[3];
+    const double rr = F.wr[0], ur = F.wr[1], pr = F.wr[3];
+    F.al = (rl > 0.0) ? sound_speed(gamma, rl, pl) : 0.0;
+    F.ar = (rr > 0.0) ? sound_speed(gamma, rr, pr) : 0.0;
+    F.ps = 0.0;
+    if (hllc_is_vacuum(gamma, rl, ul, F.al, rr, ur, F.ar)) {
+        F.kind = HLLC_VACUUM;
+        return;
+    }
+    double g;
+    if (guess_select(gamma, rl, ul, pl, F.al, rr, ur, pr, F.ar, &g)) {
+        F.kind = HLLC_TRRS;
+        F.ps = g;
+        return;
+    }
+    F.kind = HLLC_READY;
+    F.ps = (g < tol) ? tol : g;
+}
+
+HFV_HD __forceinline__ int hllc_end_branchy(double gamma, const HllcFace& F, double* f)
+{
+    if (F.kind == HLLC_VACUUM) {
+        hllc_vacuum_flux<4>(gamma, F.wl, F.al, F.wr, F.ar, f);
+        return ERR_NONE;
+    }
+    return hllc_from_guess<4>(gamma, F.wl, F.wr, F.al, F.ar, F.ps, f);
 }
 
 // After the first two powers of a two-rarefaction face (pzl = pow(pl, h/gamma),

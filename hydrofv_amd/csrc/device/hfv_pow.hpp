@@ -1,6 +1,6 @@
-// hfv_pow.hpp -- pow(x, y) for the pressures of the two-rarefaction guess:
-// positive finite normal x, finite y, results well inside the double range.
-// Other arguments fall back to the library pow.
+// hfv_pow.hpp -- pow(x, y) in double-double arithmetic, for the pressures
+// of the two-rarefaction guess (positive x, y = (gamma-1)/(2 gamma) or its
+// inverse), without a call to the library pow on any argument.
 //
 // Method (standard table-based log and exp in double-double arithmetic):
 //   x = 2^k m, m in [0.75, 1.5); c = RN(1/m_i) for m's interval i of width
@@ -12,9 +12,15 @@
 //   split so n*LN2_64_HI is exact), |r| <= ln2/128; exp(r) - 1 by its
 //   Taylor series to r^6; 2^(n/64) = 2^(n>>6) * 2^(j/64) with 2^(j/64) a
 //   double-double table entry.
-// The result before its last rounding is within ~2^-64 relative of x^y, so
-// it is the correctly rounded x^y except when x^y lies within ~2^-11 ulp of
-// a rounding boundary.  Tables: hfv_pow_tables.hpp (gen_pow_tables.py).
+// For positive finite x (subnormal included) with a normal result, the
+// value before its last rounding is within ~2^-64 relative of x^y, so it is
+// the correctly rounded x^y except when x^y lies within ~2^-11 ulp of a
+// rounding boundary.  Results in the subnormal range are rounded twice
+// (within 1 ulp), and where y log x lies within ~1e-13 below
+// ln(DBL_MAX) the result may be DBL_MAX where x^y rounds to infinity.
+// Zero, negative, infinite and NaN arguments follow C99 F.9.4.4 (pow),
+// evaluated by selects so that no lane branches.
+// Tables: hfv_pow_tables.hpp (gen_pow_tables.py).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -128,15 +134,35 @@ HFV_HD __forceinline__ double exp_dd(DD w)
     return ldexp(s.hi + low, e);
 }
 
-// x^y; the fast path covers positive finite normal x with |y log x| < 700
+// x^y for all x, y (C99 pow semantics; accuracy as stated at the top)
 HFV_HD __forceinline__ double pow_pos(double x, double y)
 {
-    if (!(x >= 0x1p-1022 && x <= 0x1.fffffffffffffp+1023)) return pow(x, y);
-    const DD l = log_dd(x);
+    const double ax = fabs(x);
+    const bool regular = (ax > 0.0) & (ax < INFINITY);  // false for 0, inf, NaN
+    const DD l = log_dd(regular ? ax : 1.0);
     const double wh = y * l.hi;
-    if (!(fabs(wh) < 700.0)) return pow(x, y);
     const double wl = fma(y, l.hi, -wh) + y * l.lo;
-    return exp_dd(DD{wh, wl});
+    // beyond these x^y rounds to infinity or to zero
+    const bool ovf = wh > 709.782712893384;
+    const bool unf = wh < -745.2;
+    const bool inrange = !(ovf | unf) & (wh == wh);
+    double r = exp_dd(DD{inrange ? wh : 0.0, inrange ? wl : 0.0});
+    r = ovf ? INFINITY : r;
+    r = unf ? 0.0 : r;
+    // |x| = 0 or infinity
+    r = (ax == 0.0) ? ((y > 0.0) ? 0.0 : INFINITY) : r;
+    r = (ax == INFINITY) ? ((y > 0.0) ? INFINITY : 0.0) : r;
+    // negative x (-0 and -infinity included): odd integer y keeps the sign,
+    // even integer y drops it, any other y gives NaN for finite nonzero x
+    const bool yint = rint(y) == y;
+    const bool yodd = yint & (rint(0.5 * y) != 0.5 * y);
+    const bool neg = signbit(x);
+    r = (neg & yodd) ? -r : r;
+    r = (neg & !yint & regular) ? NAN : r;
+    r = ((x != x) | (y != y)) ? x + y : r;
+    // x^0 = 1, 1^y = 1, (-1)^(+-inf) = 1
+    r = ((y == 0.0) | (x == 1.0) | ((ax == 1.0) & (fabs(y) == INFINITY))) ? 1.0 : r;
+    return r;
 }
 
 }  // namespace hfv

@@ -302,7 +302,7 @@ namespace {
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
-    ~DevBuf() { if (p) hipFree(p); }
+    ~DevBuf() { if (p) (void)hipFree(p); }
     bool alloc(size_t n) { return hipMalloc(&p, n * sizeof(T) + 64) == hipSuccess; }
 };
 

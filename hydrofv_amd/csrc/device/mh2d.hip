@@ -2,6 +2,8 @@
 // the per-wavefront work is in mh2d_strip.hpp).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "hfv_ctrl.hpp"
 #include "hfv_gpu_internal.hpp"
 #include "hfv_math.hpp"
@@ -9,7 +11,7 @@
 
 namespace hfv {
 
-template <int LIM, int SOLVER, bool WMODE>
+template <int LIM, int SOLVER, bool WMODE, bool BRANCHY>
 __global__ void __launch_bounds__(LANES) k_mh2d_step(Grid2D G, RunParams rp, Fields4 Ua, Fields4 Ub, Fields4 W,
                                                      Ctrl* ctrl, int slot)
 {
@@ -25,27 +27,44 @@ __global__ void __launch_bounds__(LANES) k_mh2d_step(Grid2D G, RunParams rp, Fie
     const Fields4 U = plan.cur ? Ub : Ua;   // state n
     const Fields4 V = plan.cur ? Ua : Ub;   // state n+1
     const int tile = xcd_remap(blockIdx.x, G.nsx * G.nsy);
-    const StripResult r = strip_step<LIM, SOLVER, WMODE>(G, plan.dt, U, V, W, tile % G.nsx, tile / G.nsx, lane);
+    const StripResult r =
+        strip_step<LIM, SOLVER, WMODE, BRANCHY>(G, plan.dt, U, V, W, tile % G.nsx, tile / G.nsx, lane);
     strip_finish(r, out, lane);
 }
 
-// CFL maxima of a primitive state held in memory (the first step's dt).
-__global__ void k_smax_from_w(Grid2D G, Fields4 W, Ctrl* ctrl, int slot)
+// CFL maxima of a primitive state held in memory (the first step's dt):
+// workgroups stride over the interior rows, reduce in registers and LDS, and
+// add one atomic per key (the comparisons of utils.c:75-92 skip NaN).
+constexpr int SMAX_THREADS = 256;
+
+__global__ void __launch_bounds__(SMAX_THREADS) k_smax_from_w(Grid2D G, Fields4 W, Ctrl* ctrl, int slot)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
     double sx = -INFINITY, sy = -INFINITY;
-    if (i < G.nx) {
-        const long long k = (long long)(j + G.g) * G.pitch + (i + G.g);
-        const double a = sound_speed(G.gamma, W.f[0][k], W.f[3][k]);
-        const double vx = a + fabs(W.f[1][k]);
-        const double vy = a + fabs(W.f[2][k]);
-        if (vx > sx) sx = vx;
-        if (vy > sy) sy = vy;
+    for (int j = blockIdx.x; j < G.ny; j += gridDim.x) {
+        const long long row = (long long)(j + G.g) * G.pitch + G.g;
+        for (int i = threadIdx.x; i < G.nx; i += SMAX_THREADS) {
+            const long long k = row + i;
+            const double a = sound_speed(G.gamma, W.f[0][k], W.f[3][k]);
+            const double vx = a + fabs(W.f[1][k]);
+            const double vy = a + fabs(W.f[2][k]);
+            if (vx > sx) sx = vx;
+            if (vy > sy) sy = vy;
+        }
     }
     sx = wave_max(sx);
     sy = wave_max(sy);
-    if ((threadIdx.x & 63) == 0) {
+    __shared__ double red[2][SMAX_THREADS / LANES];
+    const int w = threadIdx.x / LANES;
+    if (threadIdx.x % LANES == 0) {
+        red[0][w] = sx;
+        red[1][w] = sy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < SMAX_THREADS / LANES; q++) {
+            if (red[0][q] > sx) sx = red[0][q];
+            if (red[1][q] > sy) sy = red[1][q];
+        }
         const unsigned long long kx = smax_key(sx), ky = smax_key(sy);
         if (kx) atomicMax(&ctrl[slot].key[0], kx);
         if (ky) atomicMax(&ctrl[slot].key[1], ky);
@@ -129,32 +148,39 @@ namespace {
 
 hipStream_t pick(hfv_mh2d* h, void* stream) { return stream ? (hipStream_t)stream : h->own; }
 
-template <int LIM, int SOLVER>
+template <int LIM, int SOLVER, bool BRANCHY>
 void launch_step_t(hfv_mh2d* h, bool wmode, hipStream_t s)
 {
     const int slot = (int)(h->launches % 3);
     const dim3 grid(h->G.nsx * h->G.nsy), block(LANES);
     if (wmode) {
-        hipLaunchKernelGGL((k_mh2d_step<LIM, SOLVER, true>), grid, block, 0, s, h->G, h->rp, h->U[0], h->U[1],
-                           h->W, h->ctrl, slot);
+        hipLaunchKernelGGL((k_mh2d_step<LIM, SOLVER, true, BRANCHY>), grid, block, 0, s, h->G, h->rp, h->U[0],
+                           h->U[1], h->W, h->ctrl, slot);
     } else {
-        hipLaunchKernelGGL((k_mh2d_step<LIM, SOLVER, false>), grid, block, 0, s, h->G, h->rp, h->U[0], h->U[1],
-                           h->W, h->ctrl, slot);
+        hipLaunchKernelGGL((k_mh2d_step<LIM, SOLVER, false, BRANCHY>), grid, block, 0, s, h->G, h->rp, h->U[0],
+                           h->U[1], h->W, h->ctrl, slot);
     }
     h->launches++;
+}
+
+template <int LIM>
+void launch_hllc(hfv_mh2d* h, bool wmode, hipStream_t s)
+{
+    if (h->cfg.hllc_form == HFV_HLLC_BRANCHES) launch_step_t<LIM, HLLC, true>(h, wmode, s);
+    else launch_step_t<LIM, HLLC, false>(h, wmode, s);
 }
 
 void launch_step(hfv_mh2d* h, bool wmode, hipStream_t s)
 {
     const int lim = h->cfg.slope_limiter, sol = h->cfg.riemann_solver;
     if (sol == HLLC) {
-        if (lim == MINMOD) launch_step_t<MINMOD, HLLC>(h, wmode, s);
-        else if (lim == VAN_LEER) launch_step_t<VAN_LEER, HLLC>(h, wmode, s);
-        else launch_step_t<MC, HLLC>(h, wmode, s);
+        if (lim == MINMOD) launch_hllc<MINMOD>(h, wmode, s);
+        else if (lim == VAN_LEER) launch_hllc<VAN_LEER>(h, wmode, s);
+        else launch_hllc<MC>(h, wmode, s);
     } else {
-        if (lim == MINMOD) launch_step_t<MINMOD, EXACT>(h, wmode, s);
-        else if (lim == VAN_LEER) launch_step_t<VAN_LEER, EXACT>(h, wmode, s);
-        else launch_step_t<MC, EXACT>(h, wmode, s);
+        if (lim == MINMOD) launch_step_t<MINMOD, EXACT, false>(h, wmode, s);
+        else if (lim == VAN_LEER) launch_step_t<VAN_LEER, EXACT, false>(h, wmode, s);
+        else launch_step_t<MC, EXACT, false>(h, wmode, s);
     }
 }
 
@@ -171,6 +197,7 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
     for (int s = 0; s < 4; s++)
         if (cfg->bc[s] < BC_NONE || cfg->bc[s] > BC_PERIODIC) return HFV_EINVAL;
     if (cfg->strip_rows < 0) return HFV_EINVAL;
+    if (cfg->hllc_form != HFV_HLLC_SELECTS && cfg->hllc_form != HFV_HLLC_BRANCHES) return HFV_EINVAL;
     hfv_mh2d* h = new (std::nothrow) hfv_mh2d();
     if (!h) return HFV_ENOMEM;
     h->cfg = *cfg;
@@ -259,7 +286,7 @@ int hfv_mh2d_start(hfv_mh2d* h, void* stream)
     hipStream_t s = pick(h, stream);
     h->launches = 0;
     hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, h->ctrl, 0.0, 0LL);
-    const dim3 grid((h->G.nx + 255) / 256, h->G.ny), block(256);
+    const dim3 grid((unsigned)std::min(h->G.ny, 2048)), block(SMAX_THREADS);
     hipLaunchKernelGGL(k_smax_from_w, grid, block, 0, s, h->G, h->W, h->ctrl, 0);
     launch_step(h, true, s);  // first step from the caller's primitive state
     return hipGetLastError() == hipSuccess ? HFV_OK : HFV_EDEVICE;
@@ -283,7 +310,7 @@ int hfv_mh2d_advance(hfv_mh2d* h, long long launches, int use_graph, void* strea
             continue;
         }
         if (!h->graph || h->graph_phase != phase || h->graph_stream != s) {
-            if (h->graph) hipGraphExecDestroy(h->graph);
+            if (h->graph) (void)hipGraphExecDestroy(h->graph);
             h->graph = nullptr;
             hipGraph_t graph;
             if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return HFV_EDEVICE;
@@ -292,10 +319,10 @@ int hfv_mh2d_advance(hfv_mh2d* h, long long launches, int use_graph, void* strea
             h->launches = saved;
             if (hipStreamEndCapture(s, &graph) != hipSuccess) return HFV_EDEVICE;
             if (hipGraphInstantiate(&h->graph, graph, nullptr, nullptr, 0) != hipSuccess) {
-                hipGraphDestroy(graph);
+                (void)hipGraphDestroy(graph);
                 return HFV_EDEVICE;
             }
-            hipGraphDestroy(graph);
+            (void)hipGraphDestroy(graph);
             h->graph_phase = phase;
             h->graph_stream = s;
         }

@@ -131,8 +131,8 @@ __device__ __forceinline__ int yflux(double gamma, double tol, const double* hi,
 // The powers of the two-rarefaction pressure guesses pending on two faces per
 // lane (riemann_solver.c:191-199), evaluated 64 at a time over the
 // wavefront: the lanes' pending bases are packed into LDS, each lane raises
-// one packed value per pass (pow_pos: the fast path of hfv_pow.hpp for
-// positive pressures), and the results are read back.  A row of a
+// one packed value per pass (pow_pos, hfv_pow.hpp), and the results are
+// read back.  A row of a
 // strip has about 30 such faces in 128, so the first two powers take one or
 // two passes; the last power is needed only by the rare faces trrs_settle
 // cannot settle (about 1 in 1000 on a Kelvin-Helmholtz run), and its pass
@@ -198,8 +198,10 @@ struct StripResult {
     long long err_cell;
 };
 
-// One strip (sx, sy) of the step U -> V with time step dt.
-template <int LIM, int SOLVER, bool WMODE>
+// One strip (sx, sy) of the step U -> V with time step dt.  BRANCHY: the
+// HLLC faces by branches (hllc_begin_branchy) instead of selects; the same
+// bits either way.
+template <int LIM, int SOLVER, bool WMODE, bool BRANCHY>
 __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, const Fields4& U, const Fields4& V,
                                                   const Fields4& W, int sx, int sy, int lane)
 {
@@ -273,30 +275,45 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         if constexpr (SOLVER == HLLC) {
             // both faces' fluxes, with the wavefront's two-rarefaction
             // powers taken together
-            HllcFace Fy, Fx;
-            {
-                const double l[4] = {yt[0], yt[2], yt[1], yt[3]};
-                const double rr[4] = {ylo[0], ylo[2], ylo[1], ylo[3]};
-                hllc_begin(gamma, tol, l, rr, Fy);
+            HllcFace F[2];  // y face (normal velocity first), x face
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                const int t = (f == 1) ? 2 : (f == 2) ? 1 : f;
+                F[0].wl[f] = yt[t];
+                F[0].wr[f] = ylo[t];
+                F[1].wl[f] = hl[f];
+                F[1].wr[f] = xlo[f];
             }
-            hllc_begin(gamma, tol, hl, xlo, Fx);
-            trrs_powers(gamma, tol, Fy, Fx, trrs_q, lane);
-            // both faces' regular fluxes back to back, then the rare vacuum faces
-            double o[4];
-            ey = hllc_end_regular(gamma, Fy, o);
-            ex = hllc_end_regular(gamma, Fx, fxl);
-            if (Fy.kind == HLLC_VACUUM) {
-                hllc_vacuum_flux<4>(gamma, Fy.wl, Fy.al, Fy.wr, Fy.ar, o);
-                ey = ERR_NONE;
+            double o[2][4];
+            if constexpr (BRANCHY) {
+                hllc_begin_branchy(gamma, tol, F[0]);
+                hllc_begin_branchy(gamma, tol, F[1]);
+                trrs_powers(gamma, tol, F[0], F[1], trrs_q, lane);
+                ey = hllc_end_branchy(gamma, F[0], o[0]);
+                ex = hllc_end_branchy(gamma, F[1], o[1]);
+            } else {
+                hllc_begin_faces<2>(gamma, tol, F);
+                trrs_powers(gamma, tol, F[0], F[1], trrs_q, lane);
+                // both faces' regular fluxes together, then the rare vacuum faces
+                int e2[2];
+                hllc_end_regular_faces<2>(gamma, F, o, e2);
+                ey = e2[0];
+                ex = e2[1];
+                if (F[0].kind == HLLC_VACUUM) {
+                    hllc_vacuum_flux<4>(gamma, F[0].wl, F[0].al, F[0].wr, F[0].ar, o[0]);
+                    ey = ERR_NONE;
+                }
+                if (F[1].kind == HLLC_VACUUM) {
+                    hllc_vacuum_flux<4>(gamma, F[1].wl, F[1].al, F[1].wr, F[1].ar, o[1]);
+                    ex = ERR_NONE;
+                }
             }
-            if (Fx.kind == HLLC_VACUUM) {
-                hllc_vacuum_flux<4>(gamma, Fx.wl, Fx.al, Fx.wr, Fx.ar, fxl);
-                ex = ERR_NONE;
-            }
-            fyt[0] = o[0];
-            fyt[1] = o[2];
-            fyt[2] = o[1];
-            fyt[3] = o[3];
+            fyt[0] = o[0][0];
+            fyt[1] = o[0][2];
+            fyt[2] = o[0][1];
+            fyt[3] = o[0][3];
+#pragma unroll
+            for (int f = 0; f < 4; f++) fxl[f] = o[1][f];
         } else {
             ey = yflux<SOLVER>(gamma, tol, yt, ylo, fyt);
             ex = flux<SOLVER, 4>(gamma, tol, hl, xlo, fxl);

@@ -20,6 +20,7 @@ PROTOTYPES = {
     "hfv_ens_create": (C.c_int, [C.c_int, _P(MH2DConfig), _P(C.c_void_p)]),
     "hfv_ens_destroy": (None, [C.c_void_p]),
     "hfv_ens_set_strip_rows": (C.c_int, [C.c_void_p, _P(C.c_int)]),
+    "hfv_ens_set_hllc_form": (C.c_int, [C.c_void_p, C.c_int]),
     "hfv_ens_num_tiles": (C.c_int, [C.c_void_p]),
     "hfv_ens_upload": (C.c_int, [C.c_void_p, C.c_int, _P(gpu.dp), _P(gpu.dp), C.c_void_p]),
     "hfv_ens_start": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -77,6 +78,9 @@ class Ensemble:
         a = (C.c_int * self.count)(*[int(r) for r in rows])
         check(self.lib, self.lib.hfv_ens_set_strip_rows(self.h, a), "set_strip_rows")
 
+    def set_hllc_form(self, form):
+        check(self.lib, self.lib.hfv_ens_set_hllc_form(self.h, int(form)), "set_hllc_form")
+
     @property
     def num_tiles(self):
         return self.lib.hfv_ens_num_tiles(self.h)
@@ -122,41 +126,51 @@ def strip_rule(sides, kind, value):
 
 
 def autotune(ens: Ensemble, sides, trial_launches=8, log=None, stream=None):
-    """Pick the strip heights that minimise the ensemble step time.  Each
-    candidate rule is timed over `trial_launches` steps continuing from the
-    current device state (the step cost hardly depends on the state); the
-    batch's initial state is uploaded once afterwards."""
-    candidates = [("fixed", v) for v in (16, 32, 64)] + [("prop", d) for d in (4, 8, 16, 32)]
+    """Pick the strip heights and the HLLC code form (hfv_mh2d_config.hllc_form)
+    that minimise the ensemble step time.  Each candidate is timed over
+    `trial_launches` steps continuing from the current device state (the
+    step cost hardly depends on the state; both forms of a strip rule are
+    timed back to back); the batch's initial state is uploaded once
+    afterwards.  Returns ((form, kind, value, seconds per step, rows), all
+    results)."""
+    rules = [("fixed", v) for v in (16, 32, 64)] + [("prop", d) for d in (4, 8, 16, 32)]
+    forms = (gpu.HLLC_SELECTS, gpu.HLLC_BRANCHES) if ens.cfgs[0].riemann_solver == 2 else (gpu.HLLC_SELECTS,)
     best = None
     results = []
     ens.start(stream)
-    for kind, value in candidates:
+    for kind, value in rules:
         rows = strip_rule(sides, kind, value)
         ens.set_strip_rows(rows)
-        ens.advance(2, use_graph=False, stream=stream)
-        ens.status(stream)
-        t0 = time.perf_counter()
-        ens.advance(trial_launches, use_graph=False, stream=stream)
-        ens.status(stream)
-        dt = (time.perf_counter() - t0) / trial_launches
-        results.append((kind, value, dt, ens.num_tiles))
-        if log:
-            log(f"autotune {kind}={value}: {dt * 1e3:.3f} ms/step, {ens.num_tiles} tiles")
-        if best is None or dt < best[2]:
-            best = (kind, value, dt, rows)
-    ens.set_strip_rows(best[3])
+        for form in forms:
+            ens.set_hllc_form(form)
+            ens.advance(2, use_graph=False, stream=stream)
+            ens.status(stream)
+            t0 = time.perf_counter()
+            ens.advance(trial_launches, use_graph=False, stream=stream)
+            ens.status(stream)
+            dt = (time.perf_counter() - t0) / trial_launches
+            results.append((form, kind, value, dt, ens.num_tiles))
+            if log:
+                log(f"autotune form={FORM_NAMES[form]} {kind}={value}: {dt * 1e3:.3f} ms/step, {ens.num_tiles} tiles")
+            if best is None or dt < best[3]:
+                best = (form, kind, value, dt, rows)
+    ens.set_strip_rows(best[4])
+    ens.set_hllc_form(best[0])
     if ens.init is not None:
         ens.upload(ens.init, stream)
     return best, results
 
 
+FORM_NAMES = {gpu.HLLC_SELECTS: "selects", gpu.HLLC_BRANCHES: "branches"}
+
+
 def benchmark(lib, count=256, seed=2, steps=96, log=None):
+    """Aggregate cell-updates/s of the batched engine (autotuned strips and
+    HLLC code form, one launch per step, hipGraph replays) against the same
+    instances run one after another on the single-grid engine (the same code
+    form, its own graph replays and default strips), both timed over `steps`
+    steps from the initial state with the data resident on the device."""
     assert steps % 48 == 0, "whole graph replays only"
-    """Aggregate cell-updates/s of the batched engine (autotuned strips, one
-    launch per step, hipGraph replays) against the same instances run one
-    after another on the single-grid engine (its own graph replays and
-    default strips), both timed over `steps` steps from the initial state
-    with the data resident on the device."""
     import torch
 
     sides, cfgs, init = build_instances(lib, count=count, seed=seed, steps=-1)
@@ -182,6 +196,7 @@ def benchmark(lib, count=256, seed=2, steps=96, log=None):
 
     singles = []
     for c, (W, U) in zip(cfgs, init):
+        c.hllc_form = best[0]  # the sequential runs use the same code form
         m = gpu.MH2D(lib, c)
         m.upload(W, U, stream=sh)
         m.start(stream=sh)
@@ -200,7 +215,8 @@ def benchmark(lib, count=256, seed=2, steps=96, log=None):
         m.close()
     batch = cells * steps / t_batch
     seq = cells * steps / t_seq
-    return {"instances": count, "cells": cells, "steps": steps, "strip_rule": f"{best[0]}={best[1]}",
+    return {"instances": count, "cells": cells, "steps": steps, "strip_rule": f"{best[1]}={best[2]}",
+            "hllc_form": FORM_NAMES[best[0]],
             "batched_cell_updates_per_s": batch, "sequential_cell_updates_per_s": seq,
             "speedup": batch / seq, "batched_ms_per_step": t_batch / steps * 1e3,
             "sequential_ms_per_step": t_seq / steps * 1e3}

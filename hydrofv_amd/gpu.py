@@ -29,7 +29,11 @@ class MH2DConfig(C.Structure):
         ("max_steps", C.c_longlong),
         ("fixed_dt", C.c_double),
         ("strip_rows", C.c_int),
+        ("hllc_form", C.c_int),
     ]
+
+
+HLLC_SELECTS, HLLC_BRANCHES = 0, 1  # hfv_mh2d_config.hllc_form
 
 
 class MH1DConfig(C.Structure):

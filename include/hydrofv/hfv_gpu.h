@@ -48,7 +48,16 @@ typedef struct hfv_mh2d_config {
     long long max_steps;           /* stop after this many steps; < 0: no limit */
     double fixed_dt;               /* > 0: constant dt instead of the CFL dt */
     int strip_rows;                /* rows per wavefront strip; 0: 32, fewer for small grids */
+    int hllc_form;                 /* HFV_HLLC_SELECTS (0) or HFV_HLLC_BRANCHES (1), below */
 } hfv_mh2d_config;
+
+/* Two code forms of the HLLC flux with identical results: SELECTS evaluates
+ * every branch of the solver and selects, interleaving a cell's x and y
+ * faces (fastest where neighbouring faces take different branches, as in
+ * shear layers); BRANCHES branches per wavefront (fastest where large
+ * regions are uniform).  The ensemble's autotuner times both. */
+#define HFV_HLLC_SELECTS 0
+#define HFV_HLLC_BRANCHES 1
 
 typedef struct hfv_run_status {
     double t, dt, t_prev;
@@ -93,6 +102,8 @@ int hfv_ens_create(int count, const hfv_mh2d_config *cfgs, hfv_ens **out);
 void hfv_ens_destroy(hfv_ens *e);
 /* Strip height per instance (the tile shape the autotuner picks). */
 int hfv_ens_set_strip_rows(hfv_ens *e, const int *rows);
+/* HLLC code form of every instance (HFV_HLLC_*; initially cfgs[0].hllc_form). */
+int hfv_ens_set_hllc_form(hfv_ens *e, int form);
 int hfv_ens_num_tiles(hfv_ens *e);
 int hfv_ens_upload(hfv_ens *e, int inst, const double *const W[4], const double *const U[4], void *stream);
 /* Reset every instance's time control, first dt from W, first step. */
@@ -163,7 +174,7 @@ int hfv_adj2d_step(const hfv_adj2d_config *cfg, double *const U[4], double *cons
 int hfv_probe_flux(int nf, int solver, double gamma, double tol, long long n, const double *wl,
                    const double *wr, double *f, int *err);
 /* out[n] = x[n]^y[n]: impl 0 the device library pow, impl 1 the kernels'
- * pow for positive pressures (two-rarefaction guess, riemann_solver.c:191-199). */
+ * pow of the two-rarefaction guess (riemann_solver.c:191-199). */
 int hfv_probe_pow(int impl, long long n, const double *x, const double *y, double *out);
 /* out[n] = limit_slope(a[n], b[n]) for limiter 1/2/3 (slope_limiter.c:93-115). */
 int hfv_probe_limit(int limiter, long long n, const double *a, const double *b, double *out);

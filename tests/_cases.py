@@ -130,3 +130,32 @@ def rel_err(a, b):
     scale = np.maximum(np.abs(b[fin]).max() if fin.any() else 1.0, 1e-300)
     d = np.abs(a[fin] - b[fin]).max() if fin.any() else 0.0
     return d / scale
+
+
+def pow_special_grid():
+    """Every pairing of special and ordinary x with special and ordinary y."""
+    xs = [0.0, 1e-310, 2.5e-308, 0.5, 1.0, 2.0, 3.0, 1e300, 1.7976931348623157e308, np.inf]
+    xs = xs + [-v for v in xs] + [np.nan]
+    ys = [0.0, 0.2, 0.5, 1.0, 2.0, 3.0, 5.0, 7.0, 5.000000000000001, 1e20, 2.0 ** 53 + 2, np.inf, 2000.0]
+    ys = ys + [-v for v in ys] + [np.nan]
+    gx, gy = np.meshgrid(np.array(xs), np.array(ys))
+    return gx.ravel(), gy.ravel()
+
+
+def assert_pow_special(out, x, y):
+    """out against glibc's pow: the same zero, infinity, NaN and sign; finite
+    nonzero results within 1 ulp."""
+    import ctypes as C
+    libm = C.CDLL("libm.so.6")
+    libm.pow.restype = C.c_double
+    libm.pow.argtypes = [C.c_double, C.c_double]
+    ref = np.array([libm.pow(a, b) for a, b in zip(x.tolist(), y.tolist())])
+    same_kind = (np.isnan(out) == np.isnan(ref)) & (np.isinf(out) == np.isinf(ref)) & ((out == 0) == (ref == 0))
+    same_sign = np.isnan(ref) | (np.signbit(out) == np.signbit(ref))
+    bad = ~(same_kind & same_sign)
+    assert not bad.any(), list(zip(x[bad], y[bad], out[bad], ref[bad]))[:10]
+    fin = np.isfinite(ref) & (ref != 0)
+    with np.errstate(over="ignore"):
+        assert np.all(np.abs(out[fin] - ref[fin]) <= np.spacing(np.abs(ref[fin])))
+    special = ~fin
+    np.testing.assert_array_equal(out[special], ref[special])

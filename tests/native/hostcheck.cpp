@@ -23,7 +23,7 @@ void limits(long long n, const double* a, const double* b, double* d, double* da
 
 extern "C" {
 
-// out[n] = pow_pos(x[n], y[n]) (the kernels' power of positive pressures)
+// out[n] = pow_pos(x[n], y[n]) (the kernels' pow of the two-rarefaction guess)
 void hc_pow_pos(long long n, const double* x, const double* y, double* out)
 {
     for (long long i = 0; i < n; i++) out[i] = pow_pos(x[i], y[i]);

@@ -1,7 +1,7 @@
 """Batched ensemble engine: every instance of a batched run equals a
 single-grid device run of the same instance bit for bit, whatever tile shapes
-the batch uses, and matches the oracle to 1e-12; instances keep their own
-CFL dt."""
+the batch uses and in either HLLC code form, and matches the oracle to
+1e-12; instances keep their own CFL dt."""
 import numpy as np
 import pytest
 
@@ -29,13 +29,16 @@ def single_run(lib, cfg, init, steps):
     return st, np.stack(W), np.stack(U)
 
 
-@pytest.mark.parametrize("rows", ["default", "prop8", "fixed16"])
-def test_batch_equals_single_runs(gpulib, small_batch, rows):
+@pytest.mark.parametrize("rows,form", [("default", 0), ("prop8", 0), ("fixed16", 0), ("default", 1), ("prop8", 1)])
+def test_batch_equals_single_runs(gpulib, small_batch, rows, form):
+    """form: the batch's HLLC code form (selects 0, branches 1); the single
+    runs keep the selects, so the forms are checked against each other too."""
     sides, cfgs, init = small_batch
     ens = ensemble.Ensemble(gpulib, cfgs, init)
     if rows != "default":
         kind, value = ("prop", 8) if rows == "prop8" else ("fixed", 16)
         ens.set_strip_rows(ensemble.strip_rule(sides, kind, value))
+    ens.set_hllc_form(form)
     ens.start()
     ens.advance(60, use_graph=True)  # more launches than steps: finished instances only forward
     st = ens.status()
@@ -75,3 +78,26 @@ def _sim_like(lib, n, init):
     for name, a in zip(["mass", "momentum_x", "momentum_y", "energy"], U):
         s.field(name)[...] = a
     return s
+
+
+def test_autotune_picks_a_timed_candidate(gpulib, small_batch):
+    """The autotuner leaves the batch in its fastest configuration (strips
+    and code form) with the initial state restored, so a run after it
+    equals a plain run."""
+    sides, cfgs, init = small_batch
+    ens = ensemble.Ensemble(gpulib, cfgs, init)
+    best, results = ensemble.autotune(ens, sides, trial_launches=2)
+    assert len(results) == 14 and {r[0] for r in results} == {0, 1}
+    assert best[3] == min(r[3] for r in results)
+    ens.start()
+    ens.advance(24, use_graph=False)
+    st = ens.status()
+    ref = ensemble.Ensemble(gpulib, cfgs, init)
+    ref.start()
+    ref.advance(24, use_graph=False)
+    st_ref = ref.status()
+    for i in range(len(cfgs)):
+        assert st[i].num_steps == st_ref[i].num_steps and st[i].t == st_ref[i].t
+        assert np.array_equal(np.stack(ens.download(i)[1]), np.stack(ref.download(i)[1])), i
+    ens.close()
+    ref.close()

@@ -151,6 +151,32 @@ def test_kh_wide_strips_bitwise(gpulib):
     compare("KH 130^2 x 12", W, U, Wo, Uo)
 
 
+@pytest.mark.parametrize("case", ["kh130", "quadrants"])
+def test_hllc_forms_identical(gpulib, case):
+    """The two HLLC code forms (selects, branches) give the same bits on the
+    single-grid engine, on a shear layer and on a four-quadrant problem."""
+    from hydrofv_amd import problems, workloads
+    if case == "kh130":
+        sim, limiter = K.setup_kh(gpulib, 130), 3
+    else:
+        sim, limiter = problems.four_quadrant(gpulib, 150, workloads.ensemble_specs(1, seed=3)[1][0]), 1
+    W0, U0 = sim.primitive(), sim.conserved()
+    out = []
+    for form in (gpu.HLLC_SELECTS, gpu.HLLC_BRANCHES):
+        cfg = problems.device_config(sim, limiter=limiter, solver=2, cfl=0.8, max_steps=30, hllc_form=form)
+        m = gpu.MH2D(gpulib, cfg)
+        m.upload(W0, U0)
+        m.start()
+        m.advance(40, use_graph=True)
+        st = m.status()
+        assert st.err_kind == 0 and st.num_steps == 30
+        out.append((st.t, np.stack(m.download()[1])))
+        m.close()
+    sim.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_gravity_bitwise(gpulib):
     setup = lambda lib: K.setup_random_2d(lib, 13, 9, 2, K.BC_SETS["reflective"], 11, gravity=0.7)
     sim, W, U, Wo, Uo = run_pair(gpulib, setup, "van_leer", steps=11)
@@ -215,7 +241,7 @@ def test_hllc_invalid_wave_speed_error(gpulib):
 
 
 def test_probe_pow_pos_matches_host_build(gpulib):
-    """The kernels' pow for positive pressures gives the same bits on the
+    """The kernels' pow (pow_pos) gives the same bits on the
     device as its host build (tests/native), which test_adjoint_math checks
     against glibc and against correctly rounded values; the device library
     pow is reported beside it."""
@@ -243,5 +269,13 @@ def test_probe_pow_pos_matches_host_build(gpulib):
         L.hc_pow_pos(len(x), x, y, host)
         assert np.array_equal(dev, host)
     assert np.all(np.abs(dev - ref) <= np.spacing(np.abs(ref)))
+    # zero, negative, infinite, NaN and extreme arguments
+    sx, sy = K.pow_special_grid()
+    sdev = gpu.probe_pow(gpulib, 1, sx, sy)
+    K.assert_pow_special(sdev, sx, sy)
+    if os.path.exists(so):
+        shost = np.empty_like(sx)
+        L.hc_pow_pos(len(sx), sx, sy, shost)
+        np.testing.assert_array_equal(sdev, shost)
     print(f"pow vs glibc over {len(x)} pressure-guess arguments: pow_pos differs in {np.mean(dev != ref):.2e}, "
           f"device library pow in {np.mean(lib_pow != ref):.2e}")
