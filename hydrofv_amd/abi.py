@@ -190,6 +190,8 @@ PROTOTYPES = {
     "integrator_launch_simulation": (ErrorStatus, _SEVEN),
     "muscl_hancock_1d": (ErrorStatus, _SEVEN),
     "muscl_hancock_2d": (ErrorStatus, _SEVEN),
+    "godunov_first_order_1d": (ErrorStatus, _SEVEN),
+    "godunov_first_order_2d": (ErrorStatus, _SEVEN),
 }
 
 

@@ -198,6 +198,7 @@ int hfv_ens_create(int count, const hfv_mh2d_config* cfgs, hfv_ens** out)
         if (c.slope_limiter < 1 || c.slope_limiter > 3) return HFV_EINVAL;
         if (c.riemann_solver != HLLC && c.riemann_solver != EXACT) return HFV_EINVAL;
         if (c.hllc_form != HFV_HLLC_SELECTS && c.hllc_form != HFV_HLLC_BRANCHES) return HFV_EINVAL;
+        if (c.first_order) return HFV_EINVAL;  // the batched engine runs MUSCL-Hancock only
         for (int s = 0; s < 4; s++)
             if (c.bc[s] < BC_NONE || c.bc[s] > BC_PERIODIC) return HFV_EINVAL;
     }

@@ -23,6 +23,9 @@ namespace hfv {
 
 // flags: the reference's numeric values (slope_limiter.h, riemann_solver.h, boundary.h)
 enum Limiter : int { MINMOD = 1, VAN_LEER = 2, MC = 3 };
+// In place of a limiter: no reconstruction, the faces take the cells' own
+// states (integrator_godunov_first_order.c).
+constexpr int FIRST_ORDER = 0;
 enum Solver : int { EXACT = 1, HLLC = 2 };
 enum Bc : int { BC_NONE = 1, BC_REFLECTIVE = 2, BC_TRANSMISSIVE = 3, BC_PERIODIC = 4, BC_CUSTOM = 5 };
 

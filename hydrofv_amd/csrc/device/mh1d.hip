@@ -10,6 +10,8 @@
 // at these sizes), so any grid length is accepted.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "hfv_ctrl.hpp"
 #include "hfv_math.hpp"
 #include "hfv_gpu_internal.hpp"
@@ -82,13 +84,24 @@ __device__ __forceinline__ void geo_rate(double pre, double gamma, double rho, d
 
 // Runs steps until the run finishes or 'budget' steps have been taken (the
 // host relaunches until finished, so no single launch is unbounded).
+// LIM == FIRST_ORDER: godunov_first_order_1d (integrator_godunov_first_order.c:
+// 128-231): no reconstruction, and a failed flux ends the run with its error
+// (the serial loop's goto at :172-176), reported for the lowest failing
+// interface; the update, c2p, boundary fill and geometric source are those
+// of the MUSCL-Hancock step.
 template <int LIM, int SOLVER>
 __global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays1D A, Ctrl* ctrl, long long budget)
 {
     __shared__ double red[T1D / 64];
+    __shared__ int fail_i, fail_kind;
     const int t = threadIdx.x;
     const int g = G.g, n = G.n;
     const double gamma = G.gamma;
+    if (t == 0) {
+        fail_i = INT_MAX;
+        fail_kind = ERR_NONE;
+    }
+    __syncthreads();
 
     // CFL maximum of the interior (utils.c:28-53)
     auto cfl_max = [&]() -> double {
@@ -117,6 +130,35 @@ __global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays
             break;
         }
         const double dt = plan.dt;
+        if constexpr (LIM == FIRST_ORDER) {
+            // interface fluxes 0..n from the cells' own states (:153-171)
+            int my_i = INT_MAX, my_kind = ERR_NONE;
+            for (int i = t; i <= n; i += T1D) {
+                const int k = g + i;
+                const double l[3] = {A.w[0][k - 1], A.w[1][k - 1], A.w[2][k - 1]};
+                const double r[3] = {A.w[0][k], A.w[1][k], A.w[2][k]};
+                double f[3];
+                const int e = flux<SOLVER, 3>(gamma, G.tol, l, r, f);
+                if (e != ERR_NONE && my_i == INT_MAX) {
+                    my_i = i;
+                    my_kind = e;
+                }
+                for (int q = 0; q < 3; q++) A.fl[q][i] = f[q] * dt;
+            }
+            if (my_i != INT_MAX) atomicMin(&fail_i, my_i);
+            __syncthreads();
+            if (my_i != INT_MAX && my_i == fail_i) fail_kind = my_kind;
+            __syncthreads();
+            if (fail_i != INT_MAX) {
+                c.err_kind = fail_kind;
+                c.err_cell = g + fail_i - 1;
+                c.t_prev = c.t;
+                c.t = plan.t + dt;
+                c.dt = dt;
+                c.num_steps = plan.n + 1;
+                break;
+            }
+        } else {
         // reconstruct cells g-1 .. g+n (146-246)
         for (int i = t; i < n + 2; i += T1D) {
             const int k = g + i - 1;
@@ -144,6 +186,7 @@ __global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays
                 f[0] = f[1] = f[2] = __builtin_nan("");
             }
             for (int q = 0; q < 3; q++) A.fl[q][i] = f[q] * dt;
+        }
         }
         __syncthreads();
         // update in the reference's order: a cell first gains its left face's
@@ -314,8 +357,9 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
                  const double* mid, const double* volume, hfv_run_status* st, void* stream)
 {
     hipStream_t s = (hipStream_t)stream;
-    if (cfg->num_cells_x < 2 || cfg->num_ghost_cells_side < 2) return HFV_EINVAL;
-    if (cfg->slope_limiter < 1 || cfg->slope_limiter > 3) return HFV_EINVAL;
+    if (cfg->first_order != 0 && cfg->first_order != 1) return HFV_EINVAL;
+    if (cfg->num_cells_x < 2 || cfg->num_ghost_cells_side < (cfg->first_order ? 1 : 2)) return HFV_EINVAL;
+    if (!cfg->first_order && (cfg->slope_limiter < 1 || cfg->slope_limiter > 3)) return HFV_EINVAL;
     if (cfg->riemann_solver != HLLC && cfg->riemann_solver != EXACT) return HFV_EINVAL;
     for (int q = 0; q < 2; q++)
         if (cfg->bc[q] < BC_NONE || cfg->bc[q] > BC_PERIODIC) return HFV_EINVAL;
@@ -374,7 +418,10 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
     Ctrl c;
     for (;;) {
 #define HFV_L1D(L, S) hipLaunchKernelGGL((k_mh1d_run<L, S>), dim3(1), dim3(T1D), 0, s, G, rp, A, dctrl.p, budget)
-        if (hllc) {
+        if (cfg->first_order) {
+            if (hllc) HFV_L1D(FIRST_ORDER, HLLC);
+            else HFV_L1D(FIRST_ORDER, EXACT);
+        } else if (hllc) {
             if (lim == MINMOD) HFV_L1D(MINMOD, HLLC);
             else if (lim == VAN_LEER) HFV_L1D(VAN_LEER, HLLC);
             else HFV_L1D(MC, HLLC);

@@ -173,7 +173,10 @@ void launch_hllc(hfv_mh2d* h, bool wmode, hipStream_t s)
 void launch_step(hfv_mh2d* h, bool wmode, hipStream_t s)
 {
     const int lim = h->cfg.slope_limiter, sol = h->cfg.riemann_solver;
-    if (sol == HLLC) {
+    if (h->cfg.first_order) {
+        if (sol == HLLC) launch_hllc<FIRST_ORDER>(h, wmode, s);
+        else launch_step_t<FIRST_ORDER, EXACT, false>(h, wmode, s);
+    } else if (sol == HLLC) {
         if (lim == MINMOD) launch_hllc<MINMOD>(h, wmode, s);
         else if (lim == VAN_LEER) launch_hllc<VAN_LEER>(h, wmode, s);
         else launch_hllc<MC>(h, wmode, s);
@@ -191,8 +194,10 @@ extern "C" {
 int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
 {
     *out = nullptr;
-    if (cfg->num_cells_x < 2 || cfg->num_cells_y < 2 || cfg->num_ghost_cells_side < 2) return HFV_EINVAL;
-    if (cfg->slope_limiter < 1 || cfg->slope_limiter > 3) return HFV_EINVAL;
+    if (cfg->first_order != 0 && cfg->first_order != 1) return HFV_EINVAL;
+    if (cfg->num_cells_x < 2 || cfg->num_cells_y < 2 || cfg->num_ghost_cells_side < (cfg->first_order ? 1 : 2))
+        return HFV_EINVAL;
+    if (!cfg->first_order && (cfg->slope_limiter < 1 || cfg->slope_limiter > 3)) return HFV_EINVAL;
     if (cfg->riemann_solver != HLLC && cfg->riemann_solver != EXACT) return HFV_EINVAL;
     for (int s = 0; s < 4; s++)
         if (cfg->bc[s] < BC_NONE || cfg->bc[s] > BC_PERIODIC) return HFV_EINVAL;

@@ -200,7 +200,10 @@ struct StripResult {
 
 // One strip (sx, sy) of the step U -> V with time step dt.  BRANCHY: the
 // HLLC faces by branches (hllc_begin_branchy) instead of selects; the same
-// bits either way.
+// bits either way.  LIM == FIRST_ORDER: the first-order Godunov step of
+// integrator_godunov_first_order.c:362-497, whose faces take the cells' own
+// primitive states; its sweeps, update order and source term are those of
+// the MUSCL-Hancock step, and it needs one ghost layer, not two.
 template <int LIM, int SOLVER, bool WMODE, bool BRANCHY>
 __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, const Fields4& U, const Fields4& V,
                                                   const Fields4& W, int sx, int sy, int lane)
@@ -223,14 +226,26 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     double smx = -INFINITY, smy = -INFINITY;
     __shared__ double trrs_q[6 * LANES];
 
-    // ---- prologue: rows y0-2 .. y0+1 ------------------------------------
-    double wm2[4], wm1[4], wa[4], wb[4], wc[4];
-    load_prim<WMODE>(G, U, W, y0 - 2, col, wm2);
-    load_prim<WMODE>(G, U, W, y0 - 1, col, wm1);
-    load_prim<WMODE>(G, U, W, y0, col, wa);
-    load_prim<WMODE>(G, U, W, y0 + 1, col, wb);
+    // ---- prologue: rows y0-2 .. y0+1 (first order: y0-1 .. y0) ----------
+    double wa[4], wb[4], wc[4];
     double yt[4], fyb[4];
-    {
+    if constexpr (LIM == FIRST_ORDER) {
+        double wm1[4];
+        load_prim<WMODE>(G, U, W, y0 - 1, col, wm1);
+        load_prim<WMODE>(G, U, W, y0, col, wa);
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            wb[f] = wa[f];  // unused
+            yt[f] = wa[f];
+        }
+        const int e = yflux<SOLVER>(gamma, tol, wm1, wa, fyb);
+        if (e && out_col) { err = e; err_cell = (long long)(y0 - 1) * G.tx + col; }
+    } else {
+        double wm2[4], wm1[4];
+        load_prim<WMODE>(G, U, W, y0 - 2, col, wm2);
+        load_prim<WMODE>(G, U, W, y0 - 1, col, wm1);
+        load_prim<WMODE>(G, U, W, y0, col, wa);
+        load_prim<WMODE>(G, U, W, y0 + 1, col, wb);
         double lo[4], hi_m1[4], lo0[4];
         ypredict<LIM>(wm2, wm1, wa, cy, gamma, lo, hi_m1);   // row y0-1: top face state
         ypredict<LIM>(wm1, wa, wb, cy, gamma, lo0, yt);      // row y0: bottom and top face states
@@ -247,20 +262,26 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     }
 
     for (int r = y0; r < y1; r++) {
-        load_prim<WMODE>(G, U, W, r + 2, col, wc);
         // y: row r+1's face states (the flux through the top face of row r
-        // follows below)
-        double ylo[4], yhi[4];
-        ypredict<LIM>(wa, wb, wc, cy, gamma, ylo, yhi);
-        // x: face states of row r from the lane's neighbours
-        double wl[4], wr[4];
+        // follows below); x: the face states of row r, with the lane's
+        // neighbours taken by shuffles
+        double ylo[4], yhi[4], xlo[4], xhi[4];
+        if constexpr (LIM == FIRST_ORDER) {
+            load_prim<WMODE>(G, U, W, r + 1, col, wc);
 #pragma unroll
-        for (int f = 0; f < 4; f++) {
-            wl[f] = shfl_from_left(wa[f]);
-            wr[f] = shfl_from_right(wa[f]);
-        }
-        double xlo[4], xhi[4];
-        {
+            for (int f = 0; f < 4; f++) {
+                ylo[f] = yhi[f] = wc[f];
+                xlo[f] = xhi[f] = wa[f];
+            }
+        } else {
+            load_prim<WMODE>(G, U, W, r + 2, col, wc);
+            ypredict<LIM>(wa, wb, wc, cy, gamma, ylo, yhi);
+            double wl[4], wr[4];
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                wl[f] = shfl_from_left(wa[f]);
+                wr[f] = shfl_from_right(wa[f]);
+            }
             double d[4];
 #pragma unroll
             for (int f = 0; f < 4; f++) d[f] = limit<LIM>(wa[f] - wl[f], wr[f] - wa[f]);
@@ -367,7 +388,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
 
 #pragma unroll
         for (int f = 0; f < 4; f++) {
-            wa[f] = wb[f];
+            wa[f] = (LIM == FIRST_ORDER) ? wc[f] : wb[f];
             wb[f] = wc[f];
             yt[f] = yhi[f];
             fyb[f] = fyt[f];

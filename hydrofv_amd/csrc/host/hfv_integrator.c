@@ -1,5 +1,6 @@
 /*
  * hfv_integrator.c -- muscl_hancock_1d / muscl_hancock_2d (integrator.h:145-176)
+ * and godunov_first_order_1d / godunov_first_order_2d (integrator.h:99-130)
  * running on the MI355X.
  *
  * The reference's time loop (integrator_muscl_hancock.c:124-349, 530-1160)
@@ -84,12 +85,12 @@ static ErrorStatus common_checks(const System *system, const BoundaryConditionPa
     return make_success_error_status();
 }
 
-/* Runs the 2D loop; max_steps < 0 and fixed_dt <= 0 give the reference loop. */
-ErrorStatus hfv_muscl_hancock_2d_run(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
-                                     StoringParam *sp, Settings *settings, SimulationParam *prm,
-                                     SimulationStatus *status, long long max_steps, double fixed_dt)
+/* Runs the 2D loop; max_steps < 0 and fixed_dt <= 0 give the reference loop.
+ * first_order: the first-order Godunov scheme instead of MUSCL-Hancock. */
+static ErrorStatus run_2d(BoundaryConditionParam *bc, System *system, IntegratorParam *ip, StoringParam *sp,
+                          SimulationParam *prm, SimulationStatus *status, long long max_steps, double fixed_dt,
+                          int first_order)
 {
-    (void)settings;
     char msg[256];
     if (system->coord_sys_flag_ != COORD_SYS_CARTESIAN_2D) {
         snprintf(msg, sizeof msg, "Wrong coordinate system. Supported coordinate system: \"cartesian_2d\", got: \"%s\"",
@@ -130,6 +131,7 @@ ErrorStatus hfv_muscl_hancock_2d_run(BoundaryConditionParam *bc, System *system,
     cfg.tf = prm->tf;
     cfg.max_steps = max_steps;
     cfg.fixed_dt = fixed_dt;
+    cfg.first_order = first_order;
 
     status->num_steps = 0;
     status->dt = 0.0;
@@ -168,14 +170,19 @@ ErrorStatus hfv_muscl_hancock_2d_run(BoundaryConditionParam *bc, System *system,
     return make_success_error_status();
 }
 
-ErrorStatus hfv_muscl_hancock_1d_run(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
-                                     StoringParam *sp, Settings *settings, SimulationParam *prm,
-                                     SimulationStatus *status, long long max_steps, double fixed_dt)
+static ErrorStatus run_1d(BoundaryConditionParam *bc, System *system, IntegratorParam *ip, StoringParam *sp,
+                          SimulationParam *prm, SimulationStatus *status, long long max_steps, double fixed_dt,
+                          int first_order)
 {
-    (void)settings;
     const int f = system->coord_sys_flag_;
     if (f != COORD_SYS_CARTESIAN_1D && f != COORD_SYS_CYLINDRICAL_1D && f != COORD_SYS_SPHERICAL_1D) {
-        return WRAP_RAISE_ERROR(VALUE_ERROR, "Wrong coordinate system, expected cartesian, cylindrical or spherical 1D.");
+        /* integrator_godunov_first_order.c:39-61 */
+        char msg[256];
+        snprintf(msg, sizeof msg,
+                 "Wrong coordinate system. Supported coordinate system: \"cartesian_1d\", \"cylindrical_1d\" and "
+                 "\"spherical_1d\", got: \"%s\"",
+                 system->coord_sys ? system->coord_sys : "");
+        return WRAP_RAISE_ERROR(VALUE_ERROR, msg);
     }
     ErrorStatus e = WRAP_TRACEBACK(common_checks(system, bc, sp, 0));
     if (e.return_code != SUCCESS) return e;
@@ -197,6 +204,7 @@ ErrorStatus hfv_muscl_hancock_1d_run(BoundaryConditionParam *bc, System *system,
     cfg.tf = prm->tf;
     cfg.max_steps = max_steps;
     cfg.fixed_dt = fixed_dt;
+    cfg.first_order = first_order;
     status->num_steps = 0;
     status->dt = 0.0;
     status->t = 0.0;
@@ -211,6 +219,38 @@ ErrorStatus hfv_muscl_hancock_1d_run(BoundaryConditionParam *bc, System *system,
     return make_success_error_status();
 }
 
+ErrorStatus hfv_muscl_hancock_2d_run(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
+                                     StoringParam *sp, Settings *settings, SimulationParam *prm,
+                                     SimulationStatus *status, long long max_steps, double fixed_dt)
+{
+    (void)settings;
+    return run_2d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 0);
+}
+
+ErrorStatus hfv_muscl_hancock_1d_run(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
+                                     StoringParam *sp, Settings *settings, SimulationParam *prm,
+                                     SimulationStatus *status, long long max_steps, double fixed_dt)
+{
+    (void)settings;
+    return run_1d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 0);
+}
+
+ErrorStatus hfv_godunov_first_order_2d_run(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
+                                           StoringParam *sp, Settings *settings, SimulationParam *prm,
+                                           SimulationStatus *status, long long max_steps, double fixed_dt)
+{
+    (void)settings;
+    return run_2d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 1);
+}
+
+ErrorStatus hfv_godunov_first_order_1d_run(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
+                                           StoringParam *sp, Settings *settings, SimulationParam *prm,
+                                           SimulationStatus *status, long long max_steps, double fixed_dt)
+{
+    (void)settings;
+    return run_1d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 1);
+}
+
 ErrorStatus muscl_hancock_2d(BoundaryConditionParam *bc, System *system, IntegratorParam *ip, StoringParam *sp,
                              Settings *settings, SimulationParam *prm, SimulationStatus *status)
 {
@@ -221,4 +261,18 @@ ErrorStatus muscl_hancock_1d(BoundaryConditionParam *bc, System *system, Integra
                              Settings *settings, SimulationParam *prm, SimulationStatus *status)
 {
     return hfv_muscl_hancock_1d_run(bc, system, ip, sp, settings, prm, status, -1, 0.0);
+}
+
+ErrorStatus godunov_first_order_2d(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
+                                   StoringParam *sp, Settings *settings, SimulationParam *prm,
+                                   SimulationStatus *status)
+{
+    return hfv_godunov_first_order_2d_run(bc, system, ip, sp, settings, prm, status, -1, 0.0);
+}
+
+ErrorStatus godunov_first_order_1d(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
+                                   StoringParam *sp, Settings *settings, SimulationParam *prm,
+                                   SimulationStatus *status)
+{
+    return hfv_godunov_first_order_1d_run(bc, system, ip, sp, settings, prm, status, -1, 0.0);
 }

@@ -217,11 +217,13 @@ ErrorStatus integrator_launch_simulation(BoundaryConditionParam *bc, System *sys
         return WRAP_TRACEBACK(muscl_hancock_1d(bc, system, ip, sp, st, prm, status));
     case INTEGRATOR_MUSCL_HANCOCK_2D:
         return WRAP_TRACEBACK(muscl_hancock_2d(bc, system, ip, sp, st, prm, status));
-    case INTEGRATOR_RANDOM_CHOICE_1D:
     case INTEGRATOR_GODUNOV_FIRST_ORDER_1D:
+        return WRAP_TRACEBACK(godunov_first_order_1d(bc, system, ip, sp, st, prm, status));
     case INTEGRATOR_GODUNOV_FIRST_ORDER_2D:
+        return WRAP_TRACEBACK(godunov_first_order_2d(bc, system, ip, sp, st, prm, status));
+    case INTEGRATOR_RANDOM_CHOICE_1D:
         return WRAP_RAISE_ERROR(NOT_IMPLEMENTED_ERROR,
-                                "Only the MUSCL-Hancock integrators are provided by this library.");
+                                "The random-choice integrator is not provided by this library.");
     default:
         return WRAP_RAISE_ERROR(VALUE_ERROR, "Integrator flag not recognized.");
     }

@@ -155,7 +155,8 @@ class Simulation:
                shrink_factor=None, shrink_steps=None, entry="launch_simulation",
                max_steps=None, fixed_dt=None) -> RunResult:
         """Run the integrator.  ``max_steps`` / ``fixed_dt`` select the
-        library's step-limited / constant-dt loop (hfv_muscl_hancock_*_run)."""
+        library's step-limited / constant-dt loop (hfv_muscl_hancock_*_run,
+        hfv_godunov_first_order_*_run)."""
         ip = self.params(integrator, riemann_solver, slope_limiter, cfl, tol, shrink_factor, shrink_steps)
         sp = self.lib.get_new_storing_param()
         sp.is_storing = False
@@ -163,7 +164,8 @@ class Simulation:
         prm = abi.SimulationParam(tf=tf)
         status = abi.SimulationStatus()
         if max_steps is not None or fixed_dt is not None:
-            entry = "hfv_muscl_hancock_2d_run" if self.dim == 2 else "hfv_muscl_hancock_1d_run"
+            scheme = "godunov_first_order" if integrator.startswith("godunov_first_order") else "muscl_hancock"
+            entry = f"hfv_{scheme}_{self.dim}d_run"
         if entry != "launch_simulation":
             # direct integrator call: finalize the parameters the way
             # launch_simulation would (hydro.c:41-57)

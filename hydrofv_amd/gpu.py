@@ -30,6 +30,7 @@ class MH2DConfig(C.Structure):
         ("fixed_dt", C.c_double),
         ("strip_rows", C.c_int),
         ("hllc_form", C.c_int),
+        ("first_order", C.c_int),
     ]
 
 
@@ -49,6 +50,7 @@ class MH1DConfig(C.Structure):
         ("tf", C.c_double),
         ("max_steps", C.c_longlong),
         ("fixed_dt", C.c_double),
+        ("first_order", C.c_int),
     ]
 
 
@@ -83,6 +85,8 @@ PROTOTYPES = {
     "hfv_device_error_string": (C.c_char_p, []),
     "hfv_muscl_hancock_1d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
     "hfv_muscl_hancock_2d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
+    "hfv_godunov_first_order_1d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
+    "hfv_godunov_first_order_2d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
 }
 
 

@@ -49,6 +49,10 @@ typedef struct hfv_mh2d_config {
     double fixed_dt;               /* > 0: constant dt instead of the CFL dt */
     int strip_rows;                /* rows per wavefront strip; 0: 32, fewer for small grids */
     int hllc_form;                 /* HFV_HLLC_SELECTS (0) or HFV_HLLC_BRANCHES (1), below */
+    int first_order;               /* 1: first-order Godunov (godunov_first_order_2d,
+                                      integrator_godunov_first_order.c:270-570): no
+                                      reconstruction, slope_limiter ignored, 1 ghost
+                                      layer suffices; 0: MUSCL-Hancock */
 } hfv_mh2d_config;
 
 /* Two code forms of the HLLC flux with identical results: SELECTS evaluates
@@ -127,6 +131,9 @@ typedef struct hfv_mh1d_config {
     double tf;
     long long max_steps;
     double fixed_dt;
+    int first_order;               /* 1: first-order Godunov (godunov_first_order_1d,
+                                      integrator_godunov_first_order.c:26-268), which
+                                      reports a failed flux; 0: MUSCL-Hancock */
 } hfv_mh1d_config;
 
 /* Run the whole 1D time loop on the device in one workgroup.  W, U are the

@@ -1,5 +1,6 @@
 /*
- * hfv_oracle.c -- CPU oracle for the MUSCL-Hancock finite-volume path.
+ * hfv_oracle.c -- CPU oracle for the MUSCL-Hancock finite-volume path (and
+ * the first-order Godunov scheme that shares its fluxes).
  *
  * TEST INFRASTRUCTURE ONLY (see hfv_oracle.h).  Plain C restatement of the
  * reference algorithm; each function cites the reference file:line whose
@@ -8,7 +9,7 @@
  * Reference files (hydroFV, src/):
  *   slope_limiter.c, utils.c, riemann_solver.c, riemann_solver_hllc.c,
  *   riemann_solver_exact.c, system.c, boundary.c, source_term.c,
- *   integrator_muscl_hancock.c
+ *   integrator_muscl_hancock.c, integrator_godunov_first_order.c
  */
 #include "hfv_oracle.h"
 
@@ -1090,10 +1091,97 @@ int hfo_mh1d_step(const hfo_params *p, double dt, const double *area_x, const do
 }
 
 /* ======================================================================== */
-/* Time loops: integrator_muscl_hancock.c:124-349 (1D), 530-1160 (2D)       */
+/* First-order Godunov steps: integrator_godunov_first_order.c              */
+/* The faces take the cells' own primitive states.                          */
 /* ======================================================================== */
-static void run_loop(const hfo_params *p, hfo_run *r, int dim, const double *area_x, const double *area_y,
-                     const double *mid_x, const double *volume, double *const *w, double *const *u)
+/* 2D: 385-505 (x sweep over interior rows, then y sweep over interior
+ * columns, both from the same W), then 508-529 (c2p, BC, gravity). */
+int hfo_god2d_step(const hfo_params *p, double dt, const double *area_x, const double *area_y,
+                   const double *volume, double *const *w, double *const *u, int64_t *err_cell)
+{
+    const int g = p->g, nx = p->nx, ny = p->ny;
+    const int64_t tx = nx + 2 * g;
+    step_err err = {HFO_OK, -1};
+    for (int j = g; j < g + ny; j++) {
+        for (int i = g; i < g + nx + 1; i++) {
+            const int64_t kl = (int64_t)j * tx + (i - 1);
+            const int64_t kr = kl + 1;
+            const double wl[4] = {w[0][kl], w[1][kl], w[2][kl], w[3][kl]};
+            const double wr[4] = {w[0][kr], w[1][kr], w[2][kr], w[3][kr]};
+            double f[4];
+            /* the row loop runs on after a failure; the sweep then ends the step (441-444) */
+            record_err(&err, flux_2d(p, wl, wr, f), kl);
+            for (int q = 0; q < 4; q++) u[q][kl] -= dt * f[q] * area_x[i - 1];
+            for (int q = 0; q < 4; q++) u[q][kr] += dt * f[q] * area_x[i];
+        }
+    }
+    if (err.code == HFO_OK) {
+        for (int i = g; i < g + nx; i++) {
+            for (int j = g; j < g + ny + 1; j++) {
+                const int64_t kb = (int64_t)(j - 1) * tx + i;
+                const int64_t kt = kb + tx;
+                /* the solver sees (rho, v, u, p) and returns (mass, my, mx, E): 461-476 */
+                const double wl[4] = {w[0][kb], w[2][kb], w[1][kb], w[3][kb]};
+                const double wr[4] = {w[0][kt], w[2][kt], w[1][kt], w[3][kt]};
+                double f[4];
+                record_err(&err, flux_2d(p, wl, wr, f), kb);
+                const double fx[4] = {f[0], f[2], f[1], f[3]};
+                for (int q = 0; q < 4; q++) u[q][kb] -= dt * fx[q] * area_y[j - 1];
+                for (int q = 0; q < 4; q++) u[q][kt] += dt * fx[q] * area_y[j];
+            }
+        }
+    }
+    if (err.code != HFO_OK) {
+        if (err_cell) *err_cell = err.cell;
+        return err.code;
+    }
+    hfo_c2p(p, volume, u, w);
+    hfo_bc(p, w);
+    if (p->gravity != 0.0) {
+        hfo_gravity_2d(p, dt, u);
+        hfo_c2p(p, volume, u, w);
+        hfo_bc(p, w);
+    }
+    return HFO_OK;
+}
+
+/* 1D: 150-199 (the serial build leaves the loop at the first failed flux,
+ * before that interface's update: 174-179), then 202-220 (c2p, BC,
+ * geometric source). */
+int hfo_god1d_step(const hfo_params *p, double dt, const double *area_x, const double *mid_x,
+                   const double *volume, double *const *w, double *const *u, int64_t *err_cell)
+{
+    const int g = p->g, n = p->nx;
+    for (int i = g; i < g + n + 1; i++) {
+        const double wl[3] = {w[0][i - 1], w[1][i - 1], w[2][i - 1]};
+        const double wr[3] = {w[0][i], w[1][i], w[2][i]};
+        double f[3];
+        const int rc = flux_1d(p, wl, wr, f);
+        if (rc != HFO_OK) {
+            if (err_cell) *err_cell = i - 1;
+            return rc;
+        }
+        const double dm = f[0] * dt, dmx = f[1] * dt, de = f[2] * dt;
+        u[0][i - 1] -= dm * area_x[i - 1]; u[1][i - 1] -= dmx * area_x[i - 1]; u[2][i - 1] -= de * area_x[i - 1];
+        u[0][i] += dm * area_x[i]; u[1][i] += dmx * area_x[i]; u[2][i] += de * area_x[i];
+    }
+    hfo_c2p(p, volume, u, w);
+    hfo_bc(p, w);
+    if (p->coord == HFO_CYL1D || p->coord == HFO_SPH1D) {
+        hfo_geometry_source_1d(p, dt, mid_x, volume, w, u);
+        hfo_c2p(p, volume, u, w);
+        hfo_bc(p, w);
+    }
+    return HFO_OK;
+}
+
+/* ======================================================================== */
+/* Time loops: integrator_muscl_hancock.c:124-349 (1D), 530-1160 (2D);      */
+/* integrator_godunov_first_order.c:128-240 (1D), 363-546 (2D), the same.   */
+/* ======================================================================== */
+static void run_loop(const hfo_params *p, hfo_run *r, int dim, int first_order, const double *area_x,
+                     const double *area_y, const double *mid_x, const double *volume, double *const *w,
+                     double *const *u)
 {
     r->num_steps = 0;
     r->dt = 0.0;
@@ -1114,8 +1202,14 @@ static void run_loop(const hfo_params *p, hfo_run *r, int dim, const double *are
             r->error = HFO_VALUE_ERROR; /* "dt is zero." */
             return;
         }
-        const int rc = (dim == 2) ? hfo_mh2d_step(p, r->dt, area_x, area_y, volume, w, u, &r->error_cell)
-                                  : hfo_mh1d_step(p, r->dt, area_x, mid_x, volume, w, u, &r->error_cell);
+        int rc;
+        if (dim == 2) {
+            rc = first_order ? hfo_god2d_step(p, r->dt, area_x, area_y, volume, w, u, &r->error_cell)
+                             : hfo_mh2d_step(p, r->dt, area_x, area_y, volume, w, u, &r->error_cell);
+        } else {
+            rc = first_order ? hfo_god1d_step(p, r->dt, area_x, mid_x, volume, w, u, &r->error_cell)
+                             : hfo_mh1d_step(p, r->dt, area_x, mid_x, volume, w, u, &r->error_cell);
+        }
         if (rc != HFO_OK) {
             r->error = rc;
             return;
@@ -1128,11 +1222,23 @@ static void run_loop(const hfo_params *p, hfo_run *r, int dim, const double *are
 void hfo_mh2d_run(const hfo_params *p, hfo_run *r, const double *area_x, const double *area_y,
                   const double *volume, double *const *w, double *const *u)
 {
-    run_loop(p, r, 2, area_x, area_y, NULL, volume, w, u);
+    run_loop(p, r, 2, 0, area_x, area_y, NULL, volume, w, u);
 }
 
 void hfo_mh1d_run(const hfo_params *p, hfo_run *r, const double *area_x, const double *mid_x,
                   const double *volume, double *const *w, double *const *u)
 {
-    run_loop(p, r, 1, area_x, NULL, mid_x, volume, w, u);
+    run_loop(p, r, 1, 0, area_x, NULL, mid_x, volume, w, u);
+}
+
+void hfo_god2d_run(const hfo_params *p, hfo_run *r, const double *area_x, const double *area_y,
+                   const double *volume, double *const *w, double *const *u)
+{
+    run_loop(p, r, 2, 1, area_x, area_y, NULL, volume, w, u);
+}
+
+void hfo_god1d_run(const hfo_params *p, hfo_run *r, const double *area_x, const double *mid_x,
+                   const double *volume, double *const *w, double *const *u)
+{
+    run_loop(p, r, 1, 1, area_x, NULL, mid_x, volume, w, u);
 }

@@ -15,7 +15,8 @@
  *     hfo_limit, hfo_sound_speed, hfo_guess_p, hfo_flux_hllc_1d/2d,
  *     hfo_vacuum_1d/2d, hfo_exact_1d, hfo_flux_exact_1d, hfo_geometry,
  *     hfo_p2c, hfo_c2p, hfo_bc, hfo_dt_1d/2d, hfo_gravity_2d,
- *     hfo_geometry_source_1d, hfo_mh1d_run, hfo_mh2d_run (HLLC)
+ *     hfo_geometry_source_1d, hfo_mh1d_run, hfo_mh2d_run (HLLC),
+ *     hfo_god1d_run (HLLC, exact), hfo_god2d_run (HLLC)
  *   pinned by composition and closed forms only (no reference counterpart):
  *     hfo_flux_exact_2d (normal components equal hfo_flux_exact_1d bitwise;
  *     transverse momentum flux = mass flux x upwinded transverse velocity),
@@ -106,10 +107,22 @@ int hfo_mh2d_step(const hfo_params *p, double dt, const double *area_x, const do
 int hfo_mh1d_step(const hfo_params *p, double dt, const double *area_x, const double *mid_x,
                   const double *volume, double *const *w, double *const *u, int64_t *err_cell);
 
-/* The time loop of muscl_hancock_{1d,2d} (no storing, no progress bar). */
+/* One first-order Godunov step (integrator_godunov_first_order.c), in place;
+ * the limiter is not used. */
+int hfo_god2d_step(const hfo_params *p, double dt, const double *area_x, const double *area_y,
+                   const double *volume, double *const *w, double *const *u, int64_t *err_cell);
+int hfo_god1d_step(const hfo_params *p, double dt, const double *area_x, const double *mid_x,
+                   const double *volume, double *const *w, double *const *u, int64_t *err_cell);
+
+/* The time loop of muscl_hancock_{1d,2d} and godunov_first_order_{1d,2d}
+ * (no storing, no progress bar). */
 void hfo_mh2d_run(const hfo_params *p, hfo_run *r, const double *area_x, const double *area_y,
                   const double *volume, double *const *w, double *const *u);
 void hfo_mh1d_run(const hfo_params *p, hfo_run *r, const double *area_x, const double *mid_x,
                   const double *volume, double *const *w, double *const *u);
+void hfo_god2d_run(const hfo_params *p, hfo_run *r, const double *area_x, const double *area_y,
+                   const double *volume, double *const *w, double *const *u);
+void hfo_god1d_run(const hfo_params *p, hfo_run *r, const double *area_x, const double *mid_x,
+                   const double *volume, double *const *w, double *const *u);
 
 #endif

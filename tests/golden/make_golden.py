@@ -11,6 +11,7 @@ reference's own C sources) is driven through its public API only.  Outputs:
                       HLLC (1D, 2D) and exact (1D) fluxes and return codes
   runs_2d.npz         initial and final states of full 2D runs
   runs_1d.npz         the same for 1D runs (Sod, curvilinear, boundaries)
+  runs_godunov.npz    the same for godunov_first_order_{1d,2d} runs
   sod_exact_t0.2.npz  copy of examples/sod_shock_1d/sol_cartesian_1d_0.2.npz,
                       the reference's exact Sod solution at t = 0.2 (data)
 """
@@ -64,7 +65,7 @@ def _steps_then_ref(mk, limiter, steps, cfl, solver="riemann_solver_hllc", tol=1
     sim = mk(ref)
     W0, U0 = K.state(sim)
     grid = K.oracle_grid(sim, limiter, solver, tol=tol)
-    r = grid.run(cfl, max_steps=steps)
+    r = grid.run(cfl, max_steps=steps, first_order=integrator.startswith("godunov"))
     assert r.error == 0, r.error
     res = sim.launch(integrator, solver, limiter, cfl, r.t, tol=tol)
     assert res.num_steps == steps, (res.num_steps, steps)
@@ -116,12 +117,60 @@ def runs_1d():
     return out
 
 
+GODUNOV_SHAPES = [(13, 7, 1), (9, 11, 2)]
+
+
+def runs_godunov():
+    """First-order Godunov runs: 2D on tiny grids (1 and 2 ghost layers,
+    every boundary set, 11 steps across the CFL-shrink boundary), a KH run
+    and a gravity run; 1D shock tubes with both solvers, curvilinear
+    geometry and one ghost layer."""
+    out = {}
+    for (nx, ny, g) in GODUNOV_SHAPES:
+        for bname, bcs in K.BC_SETS.items():
+            seed = 300 + nx * 7 + ny + g
+            d = _steps_then_ref(lambda L: K.setup_random_2d(L, nx, ny, g, bcs, seed), "minmod", 11, 0.8,
+                                integrator="godunov_first_order_2d")
+            for k, v in d.items():
+                out[f"2d_{nx}x{ny}g{g}_{bname}/{k}"] = v
+    d = _steps_then_ref(lambda L: K.setup_kh(L, 64), "minmod", 40, 0.8, integrator="godunov_first_order_2d")
+    for k, v in d.items():
+        out[f"2d_kh64_40/{k}"] = v
+    d = _steps_then_ref(lambda L: K.setup_random_2d(L, 13, 9, 1, K.BC_SETS["reflective"], 12, gravity=0.7),
+                        "minmod", 11, 0.8, integrator="godunov_first_order_2d")
+    for k, v in d.items():
+        out[f"2d_gravity_13x9g1/{k}"] = v
+    ref = abi.bind(C.CDLL(O.REF_SO))
+    cases = {
+        "1d_sod400_hllc": (lambda L: K.setup_sod(L, 400), "riemann_solver_hllc", 0.9, 0.2),
+        "1d_sod200_exact": (lambda L: K.setup_sod(L, 200), "riemann_solver_exact", 0.9, 0.2),
+        "1d_cyl64_hllc": (lambda L: K.setup_sod(L, 64, coord="cylindrical_1d", bc=("reflective", "transmissive")),
+                          "riemann_solver_hllc", 0.9, 0.2),
+        "1d_sph64_exact": (lambda L: K.setup_sod(L, 64, coord="spherical_1d", bc=("reflective", "transmissive")),
+                           "riemann_solver_exact", 0.9, 0.2),
+        "1d_sod50g1_periodic": (lambda L: K.setup_sod(L, 50, g=1, bc=("periodic", "periodic")),
+                                "riemann_solver_hllc", 0.8, 0.05),
+    }
+    for key, (mk, solver, cfl, tf) in cases.items():
+        sim = mk(ref)
+        W0, U0 = K.state(sim)
+        res = sim.launch("godunov_first_order_1d", solver, "minmod", cfl, tf)
+        W1, U1 = K.state(sim)
+        for k, v in dict(W0=W0, U0=U0, W1=W1, U1=U1, t=res.t, dt=res.dt, n=res.num_steps).items():
+            out[f"{key}/{k}"] = v
+    return out
+
+
 def main():
     if not O.ref_available():
         sys.exit("build the reference first: make -C oracle ref")
+    if sys.argv[1:] == ["godunov"]:
+        np.savez_compressed(os.path.join(HERE, "runs_godunov.npz"), **runs_godunov())
+        return
     np.savez_compressed(os.path.join(HERE, "scalar_vectors.npz"), **scalar_vectors())
     np.savez_compressed(os.path.join(HERE, "runs_2d.npz"), **runs_2d())
     np.savez_compressed(os.path.join(HERE, "runs_1d.npz"), **runs_1d())
+    np.savez_compressed(os.path.join(HERE, "runs_godunov.npz"), **runs_godunov())
     src = os.path.join(REF_EXAMPLES, "sod_shock_1d", "sol_cartesian_1d_0.2.npz")
     if os.path.exists(src):
         shutil.copyfile(src, os.path.join(HERE, "sod_exact_t0.2.npz"))
