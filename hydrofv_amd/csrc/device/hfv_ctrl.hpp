@@ -39,6 +39,11 @@ struct RunParams {
     long long max_steps;  // < 0: unlimited
     double fixed_dt;      // > 0: constant dt instead of the CFL dt
     double dx, dy;
+    // Pause for a snapshot: no step starts once a step after step resume_n
+    // has ended at t >= t_pause (the storing test of
+    // integrator_muscl_hancock.c:1147-1155, evaluated after every step).
+    double t_pause = __builtin_inf();
+    long long resume_n = 0;
 };
 
 struct StepPlan {
@@ -60,7 +65,7 @@ __device__ __forceinline__ StepPlan plan_step(const Ctrl* in, const RunParams& r
     s.dt = 0.0;
     *dt_zero = 0;
     const bool finished = (in->err_kind != ERR_NONE) || !(s.t < rp.tf) ||
-                          (rp.max_steps >= 0 && s.n >= rp.max_steps);
+                          (rp.max_steps >= 0 && s.n >= rp.max_steps) || (s.n > rp.resume_n && s.t >= rp.t_pause);
     if (finished) {
         s.run = false;
         return s;

@@ -358,6 +358,7 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
 {
     hipStream_t s = (hipStream_t)stream;
     if (cfg->first_order != 0 && cfg->first_order != 1) return HFV_EINVAL;
+    if (cfg->steps_start < 0 || !(cfg->t_start >= 0.0) || cfg->t_pause < 0.0) return HFV_EINVAL;
     if (cfg->num_cells_x < 2 || cfg->num_ghost_cells_side < (cfg->first_order ? 1 : 2)) return HFV_EINVAL;
     if (!cfg->first_order && (cfg->slope_limiter < 1 || cfg->slope_limiter > 3)) return HFV_EINVAL;
     if (cfg->riemann_solver != HLLC && cfg->riemann_solver != EXACT) return HFV_EINVAL;
@@ -382,6 +383,10 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
     rp.fixed_dt = cfg->fixed_dt;
     rp.dx = cfg->dx;
     rp.dy = 0.0;
+    if (cfg->t_pause > 0.0) {
+        rp.t_pause = cfg->t_pause;
+        rp.resume_n = cfg->steps_start;
+    }
 
     const size_t tx = (size_t)G.tx;
     const size_t stride = (tx + 2 + 7) / 8 * 8;
@@ -410,6 +415,9 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
         return HFV_EDEVICE;
     Ctrl c0 = {};
     c0.err_cell = -1;
+    c0.t = cfg->t_start;
+    c0.t_prev = cfg->t_start;
+    c0.num_steps = cfg->steps_start;
     if (hipMemcpyAsync(dctrl.p, &c0, sizeof(Ctrl), hipMemcpyHostToDevice, s) != hipSuccess) return HFV_EDEVICE;
 
     const int lim = cfg->slope_limiter;
@@ -434,7 +442,8 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
         if (hipGetLastError() != hipSuccess) return HFV_EDEVICE;
         if (hipMemcpyAsync(&c, dctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s) != hipSuccess) return HFV_EDEVICE;
         if (hipStreamSynchronize(s) != hipSuccess) return HFV_EDEVICE;
-        const bool done = c.err_kind != ERR_NONE || !(c.t < rp.tf) || (rp.max_steps >= 0 && c.num_steps >= rp.max_steps);
+        const bool done = c.err_kind != ERR_NONE || !(c.t < rp.tf) || (rp.max_steps >= 0 && c.num_steps >= rp.max_steps) ||
+                          (c.num_steps > rp.resume_n && c.t >= rp.t_pause);
         if (done) break;
     }
     for (int f = 0; f < 3; f++) {
@@ -451,7 +460,8 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
     st->cur = 0;
     st->smax[0] = smax_from_key(c.key[0]);
     st->smax[1] = 0.0;
-    st->finished = 1;
+    // a run stopped at a pause has not finished
+    st->finished = (c.err_kind != ERR_NONE || !(c.t < rp.tf) || (rp.max_steps >= 0 && c.num_steps >= rp.max_steps)) ? 1 : 0;
     return HFV_OK;
 }
 

@@ -338,6 +338,16 @@ int hfv_mh2d_advance(hfv_mh2d* h, long long launches, int use_graph, void* strea
     return hipGetLastError() == hipSuccess ? HFV_OK : HFV_EDEVICE;
 }
 
+int hfv_mh2d_set_pause(hfv_mh2d* h, double t_pause, long long resume_n)
+{
+    if (!(t_pause > 0.0) || resume_n < 0) return HFV_EINVAL;
+    h->rp.t_pause = t_pause;
+    h->rp.resume_n = resume_n;
+    if (h->graph) (void)hipGraphExecDestroy(h->graph);  // its launches carry the old parameters
+    h->graph = nullptr;
+    return HFV_OK;
+}
+
 int hfv_mh2d_status(hfv_mh2d* h, hfv_run_status* st, void* stream)
 {
     hipStream_t s = pick(h, stream);

@@ -68,9 +68,7 @@ static int uniform(const double *a, long n)
 static ErrorStatus common_checks(const System *system, const BoundaryConditionParam *bc, const StoringParam *sp,
                                  int two_d)
 {
-    if (sp->is_storing) {
-        return WRAP_RAISE_ERROR(NOT_IMPLEMENTED_ERROR, "Snapshot storing is not provided by this library.");
-    }
+    (void)sp;
     const int flags[4] = {bc->boundary_condition_flag_x_min_, bc->boundary_condition_flag_x_max_,
                           bc->boundary_condition_flag_y_min_, bc->boundary_condition_flag_y_max_};
     for (int k = 0; k < (two_d ? 4 : 2); k++) {
@@ -82,6 +80,31 @@ static ErrorStatus common_checks(const System *system, const BoundaryConditionPa
     const int g = system->num_ghost_cells_side;
     if (system->num_cells_x < g || (two_d && system->num_cells_y < g))
         return WRAP_RAISE_ERROR(VALUE_ERROR, "The device integrator needs at least as many cells per direction as ghost cells.");
+    return make_success_error_status();
+}
+
+/* Snapshot bookkeeping of the reference loop (integrator_muscl_hancock.c:
+ * 509-525, 1146-1155): the initial snapshot when asked for, then one after
+ * each step that ends at t >= interval * (stored - initial + 1). */
+typedef struct {
+    int on;
+    int offset;        /* 1 when the initial state was stored */
+    long long resume;  /* steps taken when the last snapshot was written */
+} Storing;
+
+static double next_store_time(const StoringParam *sp, const Storing *s)
+{
+    return sp->storing_interval * (double)(sp->store_count_ - s->offset + 1);
+}
+
+static ErrorStatus storing_begin(Storing *s, const BoundaryConditionParam *bc, const System *system,
+                                 const IntegratorParam *ip, SimulationStatus *status, StoringParam *sp)
+{
+    s->on = sp->is_storing ? 1 : 0;
+    s->offset = (sp->is_storing && sp->store_initial) ? 1 : 0;
+    s->resume = 0;
+    if (s->on && sp->store_initial)
+        return WRAP_TRACEBACK(store_snapshot(bc, system, ip, status, sp));
     return make_success_error_status();
 }
 
@@ -137,6 +160,9 @@ static ErrorStatus run_2d(BoundaryConditionParam *bc, System *system, Integrator
     status->dt = 0.0;
     status->t = 0.0;
     sp->store_count_ = 0;
+    Storing store;
+    e = storing_begin(&store, bc, system, ip, status, sp);
+    if (e.return_code != SUCCESS) return e;
 
     hfv_mh2d *h = NULL;
     int rc = hfv_mh2d_create(&cfg, &h);
@@ -144,9 +170,37 @@ static ErrorStatus run_2d(BoundaryConditionParam *bc, System *system, Integrator
     double *W[4] = {system->density_, system->velocity_x_, system->velocity_y_, system->pressure_};
     double *U[4] = {system->mass_, system->momentum_x_, system->momentum_y_, system->energy_};
     hfv_run_status st;
+    memset(&st, 0, sizeof st);
     rc = hfv_mh2d_upload(h, (const double *const *)W, (const double *const *)U, NULL);
+    if (rc == HFV_OK && store.on) rc = hfv_mh2d_set_pause(h, next_store_time(sp, &store), 0);
     if (rc == HFV_OK) rc = hfv_mh2d_start(h, NULL);
-    if (rc == HFV_OK && max_steps >= 0) {
+    if (rc == HFV_OK && store.on) {
+        /* batches of queued launches; the device pauses after each step
+         * that is due a snapshot, which is written from the host arrays */
+        long long batch = 15;
+        for (;;) {
+            rc = hfv_mh2d_status(h, &st, NULL);
+            if (rc != HFV_OK) break;
+            if (st.err_kind == HFV_ERR_NONE && st.num_steps > store.resume && st.t >= next_store_time(sp, &store)) {
+                rc = hfv_mh2d_download(h, W, U, NULL);
+                if (rc != HFV_OK) break;
+                fill_status(status, &st);
+                e = WRAP_TRACEBACK(store_snapshot(bc, system, ip, status, sp));
+                if (e.return_code != SUCCESS) {
+                    hfv_mh2d_destroy(h);
+                    return e;
+                }
+                store.resume = st.num_steps;
+                rc = hfv_mh2d_set_pause(h, next_store_time(sp, &store), store.resume);
+                if (rc != HFV_OK) break;
+                continue;
+            }
+            if (st.finished) break;
+            rc = hfv_mh2d_advance(h, batch, 1, NULL);
+            if (rc != HFV_OK) break;
+            batch = (batch < 48) ? 48 : (batch < 480 ? 2 * batch : 480);
+        }
+    } else if (rc == HFV_OK && max_steps >= 0) {
         /* known step count: enqueue every launch, then check once */
         if (max_steps > 1) rc = hfv_mh2d_advance(h, max_steps - 1, 1, NULL);
         if (rc == HFV_OK) rc = hfv_mh2d_status(h, &st, NULL);
@@ -209,11 +263,28 @@ static ErrorStatus run_1d(BoundaryConditionParam *bc, System *system, Integrator
     status->dt = 0.0;
     status->t = 0.0;
     sp->store_count_ = 0;
+    Storing store;
+    e = storing_begin(&store, bc, system, ip, status, sp);
+    if (e.return_code != SUCCESS) return e;
     double *W[3] = {system->density_, system->velocity_x_, system->pressure_};
     double *U[3] = {system->mass_, system->momentum_x_, system->energy_};
     hfv_run_status st;
-    const int rc = hfv_mh1d_run(&cfg, W, U, system->surface_area_x_, system->mid_points_x_, system->volume_, &st, NULL);
-    if (rc != HFV_OK) return device_error(rc);
+    for (;;) {
+        /* with storing, the run goes in segments that end at the steps due
+         * a snapshot and resume from the state written back to the host */
+        if (store.on) cfg.t_pause = next_store_time(sp, &store);
+        const int rc =
+            hfv_mh1d_run(&cfg, W, U, system->surface_area_x_, system->mid_points_x_, system->volume_, &st, NULL);
+        if (rc != HFV_OK) return device_error(rc);
+        if (!store.on || st.err_kind != HFV_ERR_NONE || !(st.num_steps > store.resume && st.t >= cfg.t_pause)) break;
+        fill_status(status, &st);
+        e = WRAP_TRACEBACK(store_snapshot(bc, system, ip, status, sp));
+        if (e.return_code != SUCCESS) return e;
+        store.resume = st.num_steps;
+        if (st.finished) break;
+        cfg.t_start = st.t;
+        cfg.steps_start = st.num_steps;
+    }
     fill_status(status, &st);
     if (st.err_kind != HFV_ERR_NONE) return run_error(st.err_kind);
     return make_success_error_status();

@@ -189,15 +189,7 @@ StoringParam get_new_storing_param(void)
     return p;
 }
 
-ErrorStatus finalize_storing_param(StoringParam *p)
-{
-    if (!p->is_storing) return make_success_error_status();
-    if (p->storing_interval <= 0.0) return WRAP_RAISE_ERROR(VALUE_ERROR, "Storing interval must be positive.");
-    if (p->output_dir && p->output_dir[0] && p->output_dir[strlen(p->output_dir) - 1] != '/')
-        return WRAP_RAISE_ERROR(VALUE_ERROR,
-                                "Directory path for storing snapshots must end with a trailing slash (\"/\").");
-    return make_success_error_status();
-}
+/* finalize_storing_param: hfv_storing.c */
 
 /* ---- utils.c:16-21 ----------------------------------------------------- */
 double hydro_get_current_time(void)

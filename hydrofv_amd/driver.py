@@ -153,13 +153,20 @@ class Simulation:
 
     def launch(self, integrator, riemann_solver, slope_limiter, cfl, tf, tol=1e-6,
                shrink_factor=None, shrink_steps=None, entry="launch_simulation",
-               max_steps=None, fixed_dt=None) -> RunResult:
+               max_steps=None, fixed_dt=None, store_dir=None, store_interval=None,
+               store_initial=False) -> RunResult:
         """Run the integrator.  ``max_steps`` / ``fixed_dt`` select the
         library's step-limited / constant-dt loop (hfv_muscl_hancock_*_run,
-        hfv_godunov_first_order_*_run)."""
+        hfv_godunov_first_order_*_run).  ``store_dir`` (ending in "/") turns
+        on snapshots every ``store_interval`` of simulation time."""
         ip = self.params(integrator, riemann_solver, slope_limiter, cfl, tol, shrink_factor, shrink_steps)
         sp = self.lib.get_new_storing_param()
-        sp.is_storing = False
+        sp.is_storing = store_dir is not None
+        if store_dir is not None:
+            self._store_dir = store_dir.encode()
+            sp.output_dir = self._store_dir
+            sp.storing_interval = float(store_interval)
+            sp.store_initial = bool(store_initial)
         st = abi.Settings(verbose=0, no_progress_bar=True)
         prm = abi.SimulationParam(tf=tf)
         status = abi.SimulationStatus()

@@ -88,6 +88,11 @@ int hfv_mh2d_start(hfv_mh2d *h, void *stream);
  * finished (t >= tf, max_steps reached, or a failure).  use_graph replays a
  * captured hipGraph of 48 launches. */
 int hfv_mh2d_advance(hfv_mh2d *h, long long launches, int use_graph, void *stream);
+/* Pause for a snapshot (storing.c via integrator_muscl_hancock.c:1147-1155):
+ * once a step after step resume_n ends at t >= t_pause, further launches
+ * only forward the state until the pause is moved.  Call between batches
+ * (after hfv_mh2d_status); t_pause = INFINITY lifts it. */
+int hfv_mh2d_set_pause(hfv_mh2d *h, double t_pause, long long resume_n);
 /* Synchronise the stream and read the time-control state. */
 int hfv_mh2d_status(hfv_mh2d *h, hfv_run_status *st, void *stream);
 /* Primitive (c2p + ghost fill) and conserved state back to host arrays;
@@ -134,6 +139,13 @@ typedef struct hfv_mh1d_config {
     int first_order;               /* 1: first-order Godunov (godunov_first_order_1d,
                                       integrator_godunov_first_order.c:26-268), which
                                       reports a failed flux; 0: MUSCL-Hancock */
+    /* Resuming and pausing, for snapshots: the run starts at time t_start
+     * after steps_start steps (0, 0: a fresh run; W must then be the state
+     * the run left), and with t_pause > 0 it stops after the first step
+     * that ends at t >= t_pause. */
+    double t_start;
+    long long steps_start;
+    double t_pause;
 } hfv_mh1d_config;
 
 /* Run the whole 1D time loop on the device in one workgroup.  W, U are the

@@ -254,6 +254,11 @@ ErrorStatus hfv_godunov_first_order_2d_run(HFV_INTEGRATOR_ARGS, long long max_st
 /* ---- storing.h:21-49 --------------------------------------------------- */
 StoringParam get_new_storing_param(void);                            /* storing.h:21 */
 ErrorStatus finalize_storing_param(StoringParam *storing_param);     /* storing.h:30 */
+/* storing.h:43-49: <output_dir>snapshot_<store_count_>.h5 in the reference's
+ * HDF5 layout; the HDF5 C library is loaded at the first call. */
+ErrorStatus store_snapshot(const BoundaryConditionParam *boundary_condition_param, const System *system,
+                           const IntegratorParam *integrator_param, const SimulationStatus *simulation_status,
+                           StoringParam *storing_param);
 
 /* ---- utils.h:25-69 ----------------------------------------------------- */
 double hydro_get_current_time(void);                                 /* utils.h:25 */
