@@ -363,7 +363,9 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
     if (!cfg->first_order && (cfg->slope_limiter < 1 || cfg->slope_limiter > 3)) return HFV_EINVAL;
     if (cfg->riemann_solver != HLLC && cfg->riemann_solver != EXACT) return HFV_EINVAL;
     for (int q = 0; q < 2; q++)
-        if (cfg->bc[q] < BC_NONE || cfg->bc[q] > BC_PERIODIC) return HFV_EINVAL;
+        // BC_CUSTOM: the device leaves those ghosts as c2p of their own
+        // state; the host applies the hook between steps (hfv_integrator.c)
+        if (cfg->bc[q] < BC_NONE || cfg->bc[q] > BC_CUSTOM) return HFV_EINVAL;
     Grid1D G;
     G.n = cfg->num_cells_x;
     G.g = cfg->num_ghost_cells_side;

@@ -75,13 +75,13 @@ __global__ void __launch_bounds__(SMAX_THREADS) k_smax_from_w(Grid2D G, Fields4 
 // fill (x-ghosts on interior rows, y-ghosts on interior columns; the corner
 // blocks keep c2p of their own conserved state), i.e. system.c:763-802
 // followed by boundary.c:513-777.
-__global__ void k_prim_from_u(Grid2D G, Fields4 U, Fields4 W)
+__global__ void k_prim_from_u(Grid2D G, Fields4 U, Fields4 W, int fill)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int q = blockIdx.y;
     if (c >= G.tx) return;
-    const bool gx = (c < G.g) || (c >= G.g + G.nx);
-    const bool gy = (q < G.g) || (q >= G.g + G.ny);
+    const bool gx = fill && ((c < G.g) || (c >= G.g + G.nx));
+    const bool gy = fill && ((q < G.g) || (q >= G.g + G.ny));
     int cs = c, qs = q, fu = 0, fv = 0;
     if (gx && !gy) cs = map_index(c, G.nx, G.g, G.bc[0], G.bc[1], &fu);
     if (gy && !gx) qs = map_index(q, G.ny, G.g, G.bc[2], G.bc[3], &fv);
@@ -97,10 +97,20 @@ __global__ void k_prim_from_u(Grid2D G, Fields4 U, Fields4 W)
     W.f[3][k] = w[3];
 }
 
-void prim_from_u(const Grid2D& G, const Fields4& U, const Fields4& W, hipStream_t s)
+// fill = 0: c2p of every cell's own conserved state (system.c:763-802) with
+// no boundary fill
+void prim_from_u(const Grid2D& G, const Fields4& U, const Fields4& W, hipStream_t s, int fill)
 {
     const dim3 grid((G.tx + 63) / 64, G.ty), block(64);
-    hipLaunchKernelGGL(k_prim_from_u, grid, block, 0, s, G, U, W);
+    hipLaunchKernelGGL(k_prim_from_u, grid, block, 0, s, G, U, W, fill);
+}
+
+__global__ void k_clear_keys(Ctrl* ctrl, int slot)
+{
+    if (threadIdx.x == 0) {
+        ctrl[slot].key[0] = 0ull;
+        ctrl[slot].key[1] = 0ull;
+    }
 }
 
 // controller initialisation for a run: slot 0 = t 0 / step 0 / buffer 0,
@@ -142,6 +152,7 @@ struct hfv_mh2d {
     int graph_phase;
     hipStream_t graph_stream;
     hipStream_t own;       // used when the caller passes no stream (graphs cannot be captured on the null stream)
+    int w_pending;         // the next step reads the primitive state set by hfv_mh2d_set_primitive
 };
 
 namespace {
@@ -200,7 +211,9 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
     if (!cfg->first_order && (cfg->slope_limiter < 1 || cfg->slope_limiter > 3)) return HFV_EINVAL;
     if (cfg->riemann_solver != HLLC && cfg->riemann_solver != EXACT) return HFV_EINVAL;
     for (int s = 0; s < 4; s++)
-        if (cfg->bc[s] < BC_NONE || cfg->bc[s] > BC_PERIODIC) return HFV_EINVAL;
+        // BC_CUSTOM: the device leaves those ghosts as c2p of their own
+        // state; the host applies the hook between steps (hfv_integrator.c)
+        if (cfg->bc[s] < BC_NONE || cfg->bc[s] > BC_CUSTOM) return HFV_EINVAL;
     if (cfg->strip_rows < 0) return HFV_EINVAL;
     if (cfg->hllc_form != HFV_HLLC_SELECTS && cfg->hllc_form != HFV_HLLC_BRANCHES) return HFV_EINVAL;
     hfv_mh2d* h = new (std::nothrow) hfv_mh2d();
@@ -300,6 +313,11 @@ int hfv_mh2d_start(hfv_mh2d* h, void* stream)
 int hfv_mh2d_advance(hfv_mh2d* h, long long launches, int use_graph, void* stream)
 {
     hipStream_t s = pick(h, stream);
+    if (h->w_pending && launches > 0) {
+        launch_step(h, true, s);
+        h->w_pending = 0;
+        launches--;
+    }
     if (!use_graph) {
         for (long long i = 0; i < launches; i++) launch_step(h, false, s);
         return hipGetLastError() == hipSuccess ? HFV_OK : HFV_EDEVICE;
@@ -378,7 +396,8 @@ int hfv_mh2d_download(hfv_mh2d* h, double* const W[4], double* const U[4], void*
     if (rc) return rc;
     const Grid2D& G = h->G;
     const Fields4 cur = h->U[st.cur];
-    prim_from_u(G, cur, h->W, s);
+    // only when asked for: W may hold a state set by hfv_mh2d_set_primitive
+    if (W) prim_from_u(G, cur, h->W, s);
     const size_t hp = (size_t)G.tx * sizeof(double), dp = (size_t)G.pitch * sizeof(double);
     for (int f = 0; f < 4; f++) {
         if (W && hipMemcpy2DAsync(W[f], hp, h->W.f[f], dp, hp, G.ty, hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -386,6 +405,39 @@ int hfv_mh2d_download(hfv_mh2d* h, double* const W[4], double* const U[4], void*
         if (U && hipMemcpy2DAsync(U[f], hp, cur.f[f], dp, hp, G.ty, hipMemcpyDeviceToHost, s) != hipSuccess)
             return HFV_EDEVICE;
     }
+    return hipStreamSynchronize(s) == hipSuccess ? HFV_OK : HFV_EDEVICE;
+}
+
+int hfv_mh2d_primitive(hfv_mh2d* h, double* const W[4], void* stream)
+{
+    hipStream_t s = pick(h, stream);
+    hfv_run_status st;
+    int rc = hfv_mh2d_status(h, &st, stream);
+    if (rc) return rc;
+    const Grid2D& G = h->G;
+    prim_from_u(G, h->U[st.cur], h->W, s, 0);
+    const size_t hp = (size_t)G.tx * sizeof(double), dp = (size_t)G.pitch * sizeof(double);
+    for (int f = 0; f < 4; f++)
+        if (hipMemcpy2DAsync(W[f], hp, h->W.f[f], dp, hp, G.ty, hipMemcpyDeviceToHost, s) != hipSuccess)
+            return HFV_EDEVICE;
+    return hipStreamSynchronize(s) == hipSuccess ? HFV_OK : HFV_EDEVICE;
+}
+
+int hfv_mh2d_set_primitive(hfv_mh2d* h, const double* const W[4], void* stream)
+{
+    hipStream_t s = pick(h, stream);
+    const Grid2D& G = h->G;
+    const size_t hp = (size_t)G.tx * sizeof(double), dp = (size_t)G.pitch * sizeof(double);
+    for (int f = 0; f < 4; f++)
+        if (hipMemcpy2DAsync(h->W.f[f], dp, W[f], hp, hp, G.ty, hipMemcpyHostToDevice, s) != hipSuccess)
+            return HFV_EDEVICE;
+    // the next step's CFL maxima come from this state (utils.c:55-95)
+    const int slot = (int)(h->launches % 3);
+    hipLaunchKernelGGL(k_clear_keys, dim3(1), dim3(64), 0, s, h->ctrl, slot);
+    const dim3 grid((unsigned)std::min(G.ny, 2048)), block(SMAX_THREADS);
+    hipLaunchKernelGGL(k_smax_from_w, grid, block, 0, s, G, h->W, h->ctrl, slot);
+    h->w_pending = 1;
+    if (hipGetLastError() != hipSuccess) return HFV_EDEVICE;
     return hipStreamSynchronize(s) == hipSuccess ? HFV_OK : HFV_EDEVICE;
 }
 

@@ -72,15 +72,21 @@ static ErrorStatus common_checks(const System *system, const BoundaryConditionPa
     const int flags[4] = {bc->boundary_condition_flag_x_min_, bc->boundary_condition_flag_x_max_,
                           bc->boundary_condition_flag_y_min_, bc->boundary_condition_flag_y_max_};
     for (int k = 0; k < (two_d ? 4 : 2); k++) {
-        if (flags[k] == BOUNDARY_CONDITION_CUSTOM)
-            return WRAP_RAISE_ERROR(NOT_IMPLEMENTED_ERROR, "Custom boundary conditions are not supported on the device.");
-        if (flags[k] < BOUNDARY_CONDITION_NONE || flags[k] > BOUNDARY_CONDITION_PERIODIC)
+        if (flags[k] < BOUNDARY_CONDITION_NONE || flags[k] > BOUNDARY_CONDITION_CUSTOM)
             return WRAP_RAISE_ERROR(VALUE_ERROR, "Boundary condition flag not recognized.");
     }
     const int g = system->num_ghost_cells_side;
     if (system->num_cells_x < g || (two_d && system->num_cells_y < g))
         return WRAP_RAISE_ERROR(VALUE_ERROR, "The device integrator needs at least as many cells per direction as ghost cells.");
     return make_success_error_status();
+}
+
+static int has_custom_bc(const BoundaryConditionParam *bc, int two_d)
+{
+    return bc->boundary_condition_flag_x_min_ == BOUNDARY_CONDITION_CUSTOM ||
+           bc->boundary_condition_flag_x_max_ == BOUNDARY_CONDITION_CUSTOM ||
+           (two_d && (bc->boundary_condition_flag_y_min_ == BOUNDARY_CONDITION_CUSTOM ||
+                      bc->boundary_condition_flag_y_max_ == BOUNDARY_CONDITION_CUSTOM));
 }
 
 /* Snapshot bookkeeping of the reference loop (integrator_muscl_hancock.c:
@@ -174,15 +180,33 @@ static ErrorStatus run_2d(BoundaryConditionParam *bc, System *system, Integrator
     rc = hfv_mh2d_upload(h, (const double *const *)W, (const double *const *)U, NULL);
     if (rc == HFV_OK && store.on) rc = hfv_mh2d_set_pause(h, next_store_time(sp, &store), 0);
     if (rc == HFV_OK) rc = hfv_mh2d_start(h, NULL);
-    if (rc == HFV_OK && store.on) {
+    const int custom = has_custom_bc(bc, 1);
+    if (rc == HFV_OK && (store.on || custom)) {
         /* batches of queued launches; the device pauses after each step
-         * that is due a snapshot, which is written from the host arrays */
-        long long batch = 15;
+         * that is due a snapshot, which is written from the host arrays.
+         * With a custom boundary hook every step goes one at a time: the
+         * primitive state comes to the host, the reference's boundary fill
+         * runs there (hooks included, boundary.c:513-777) and the filled
+         * state goes back for the next step. */
+        long long batch = 15, filled = 0;
         for (;;) {
             rc = hfv_mh2d_status(h, &st, NULL);
             if (rc != HFV_OK) break;
-            if (st.err_kind == HFV_ERR_NONE && st.num_steps > store.resume && st.t >= next_store_time(sp, &store)) {
-                rc = hfv_mh2d_download(h, W, U, NULL);
+            if (custom && st.err_kind == HFV_ERR_NONE && st.num_steps > filled) {
+                rc = hfv_mh2d_primitive(h, W, NULL);
+                if (rc != HFV_OK) break;
+                e = WRAP_TRACEBACK(set_boundary_condition(bc, system));
+                if (e.return_code != SUCCESS) {
+                    hfv_mh2d_destroy(h);
+                    return e;
+                }
+                rc = hfv_mh2d_set_primitive(h, (const double *const *)W, NULL);
+                if (rc != HFV_OK) break;
+                filled = st.num_steps;
+            }
+            if (store.on && st.err_kind == HFV_ERR_NONE && st.num_steps > store.resume &&
+                st.t >= next_store_time(sp, &store)) {
+                rc = hfv_mh2d_download(h, custom ? NULL : W, U, NULL);
                 if (rc != HFV_OK) break;
                 fill_status(status, &st);
                 e = WRAP_TRACEBACK(store_snapshot(bc, system, ip, status, sp));
@@ -196,7 +220,7 @@ static ErrorStatus run_2d(BoundaryConditionParam *bc, System *system, Integrator
                 continue;
             }
             if (st.finished) break;
-            rc = hfv_mh2d_advance(h, batch, 1, NULL);
+            rc = custom ? hfv_mh2d_advance(h, 1, 0, NULL) : hfv_mh2d_advance(h, batch, 1, NULL);
             if (rc != HFV_OK) break;
             batch = (batch < 48) ? 48 : (batch < 480 ? 2 * batch : 480);
         }
@@ -216,7 +240,8 @@ static ErrorStatus run_2d(BoundaryConditionParam *bc, System *system, Integrator
             batch = (batch < 48) ? 48 : (batch < 480 ? 2 * batch : 480);
         }
     }
-    if (rc == HFV_OK) rc = hfv_mh2d_download(h, W, U, NULL);
+    /* with custom hooks the host arrays already hold the filled primitive state */
+    if (rc == HFV_OK) rc = hfv_mh2d_download(h, custom && st.err_kind == HFV_ERR_NONE ? NULL : W, U, NULL);
     hfv_mh2d_destroy(h);
     if (rc != HFV_OK) return device_error(rc);
     fill_status(status, &st);
@@ -269,19 +294,32 @@ static ErrorStatus run_1d(BoundaryConditionParam *bc, System *system, Integrator
     double *W[3] = {system->density_, system->velocity_x_, system->pressure_};
     double *U[3] = {system->mass_, system->momentum_x_, system->energy_};
     hfv_run_status st;
+    const int custom = has_custom_bc(bc, 0);
     for (;;) {
-        /* with storing, the run goes in segments that end at the steps due
-         * a snapshot and resume from the state written back to the host */
+        /* With storing, the run goes in segments that end at the steps due
+         * a snapshot and resume from the state written back to the host.
+         * With a custom boundary hook each segment is one step, after which
+         * the reference's boundary fill runs on the host (hooks included,
+         * boundary.c:411-511). */
         if (store.on) cfg.t_pause = next_store_time(sp, &store);
+        if (custom) cfg.max_steps = (max_steps < 0 || cfg.steps_start + 1 < max_steps) ? cfg.steps_start + 1 : max_steps;
         const int rc =
             hfv_mh1d_run(&cfg, W, U, system->surface_area_x_, system->mid_points_x_, system->volume_, &st, NULL);
         if (rc != HFV_OK) return device_error(rc);
-        if (!store.on || st.err_kind != HFV_ERR_NONE || !(st.num_steps > store.resume && st.t >= cfg.t_pause)) break;
-        fill_status(status, &st);
-        e = WRAP_TRACEBACK(store_snapshot(bc, system, ip, status, sp));
-        if (e.return_code != SUCCESS) return e;
-        store.resume = st.num_steps;
-        if (st.finished) break;
+        if (st.err_kind != HFV_ERR_NONE || st.num_steps == cfg.steps_start) break;
+        if (custom) {
+            e = WRAP_TRACEBACK(set_boundary_condition(bc, system));
+            if (e.return_code != SUCCESS) return e;
+        }
+        const int due = store.on && st.num_steps > store.resume && st.t >= cfg.t_pause;
+        if (due) {
+            fill_status(status, &st);
+            e = WRAP_TRACEBACK(store_snapshot(bc, system, ip, status, sp));
+            if (e.return_code != SUCCESS) return e;
+            store.resume = st.num_steps;
+        }
+        const int done = !(st.t < prm->tf) || (max_steps >= 0 && st.num_steps >= max_steps);
+        if (done || (!due && !custom)) break;
         cfg.t_start = st.t;
         cfg.steps_start = st.num_steps;
     }

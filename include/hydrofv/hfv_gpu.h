@@ -96,8 +96,16 @@ int hfv_mh2d_set_pause(hfv_mh2d *h, double t_pause, long long resume_n);
 /* Synchronise the stream and read the time-control state. */
 int hfv_mh2d_status(hfv_mh2d *h, hfv_run_status *st, void *stream);
 /* Primitive (c2p + ghost fill) and conserved state back to host arrays;
- * either pointer array may be NULL. */
+ * either pointer array may be NULL (with W NULL the device's primitive
+ * buffer is left as it is). */
 int hfv_mh2d_download(hfv_mh2d *h, double *const W[4], double *const U[4], void *stream);
+/* Host-side boundary conditions (custom hooks, boundary.h:17-33): the
+ * primitive state of every cell, c2p of its own conserved state with no
+ * boundary fill (system.c:763-802), copied to the host; and a primitive
+ * state, ghost cells filled by the caller, that the next step reads in place
+ * of the device's own fill, its CFL maxima included. */
+int hfv_mh2d_primitive(hfv_mh2d *h, double *const W[4], void *stream);
+int hfv_mh2d_set_primitive(hfv_mh2d *h, const double *const W[4], void *stream);
 /* Device addresses of the two conserved buffers and the primitive buffer. */
 int hfv_mh2d_fields(hfv_mh2d *h, double **U0, double **U1, double **W, long long *pitch);
 

@@ -720,27 +720,23 @@ void hfo_c2p(const hfo_params *p, const double *volume, double *const *u, double
 /* x-ghosts are filled on interior rows only and y-ghosts on interior       */
 /* columns only, so the corner blocks are never written.                    */
 /* ======================================================================== */
-static void bc_line(int flag_lo, int flag_hi, double *const *w, int nf, int normal,
-                    int64_t base, int64_t stride, int n, int g)
+/* one side of a line of cells; cell k of the line lives at base + k*stride,
+ * k in [0, n+2g).  A periodic pair is filled with the high side, both ghost
+ * strips at once (boundary.c:454-457, 477-490). */
+static void bc_side(int hi, int flag, double *const *w, int nf, int normal, int64_t base, int64_t stride, int n,
+                    int g)
 {
-    /* cell k of the line lives at base + k*stride, k in [0, n+2g) */
 #define AT(f, k) w[f][base + (int64_t)(k) * stride]
-    if (flag_lo == HFO_BC_REFLECTIVE || flag_lo == HFO_BC_TRANSMISSIVE) {
+    if (flag == HFO_BC_REFLECTIVE || flag == HFO_BC_TRANSMISSIVE) {
         for (int i = 0; i < g; i++) {
             for (int f = 0; f < nf; f++) {
-                const double s = AT(f, g + i);
-                AT(f, g - 1 - i) = (flag_lo == HFO_BC_REFLECTIVE && f == normal) ? -s : s;
+                const double s = hi ? AT(f, g + n - 1 - i) : AT(f, g + i);
+                const double v = (flag == HFO_BC_REFLECTIVE && f == normal) ? -s : s;
+                if (hi) AT(f, g + n + i) = v;
+                else AT(f, g - 1 - i) = v;
             }
         }
-    }
-    if (flag_hi == HFO_BC_REFLECTIVE || flag_hi == HFO_BC_TRANSMISSIVE) {
-        for (int i = 0; i < g; i++) {
-            for (int f = 0; f < nf; f++) {
-                const double s = AT(f, g + n - 1 - i);
-                AT(f, g + n + i) = (flag_hi == HFO_BC_REFLECTIVE && f == normal) ? -s : s;
-            }
-        }
-    } else if (flag_hi == HFO_BC_PERIODIC) {
+    } else if (hi && flag == HFO_BC_PERIODIC) {
         for (int i = 0; i < g; i++) {
             for (int f = 0; f < nf; f++) {
                 AT(f, g + n + i) = AT(f, g + i);
@@ -751,20 +747,32 @@ static void bc_line(int flag_lo, int flag_hi, double *const *w, int nf, int norm
 #undef AT
 }
 
+/* boundary.c:411-777: x_min, x_max, then y_min, y_max, each side complete
+ * before the next; a custom side calls its hook (boundary.h:17-33) */
 void hfo_bc(const hfo_params *p, double *const *w)
 {
     const int g = p->g, nx = p->nx;
     if (p->coord != HFO_CART2D) {
-        bc_line(p->bc[0], p->bc[1], w, 3, 1, 0, 1, nx, g);
+        for (int hi = 0; hi < 2; hi++) {
+            if (p->bc[hi] == HFO_BC_CUSTOM) {
+                if (p->hook[hi]) ((hfo_bc1d_fn)p->hook[hi])(w[0], w[1], w[2], g, nx);
+            } else {
+                bc_side(hi, p->bc[hi], w, 3, 1, 0, 1, nx, g);
+            }
+        }
         return;
     }
     const int ny = p->ny;
     const int64_t tx = nx + 2 * g;
-    for (int j = g; j < g + ny; j++) {
-        bc_line(p->bc[0], p->bc[1], w, 4, 1, (int64_t)j * tx, 1, nx, g);
-    }
-    for (int i = g; i < g + nx; i++) {
-        bc_line(p->bc[2], p->bc[3], w, 4, 2, i, tx, ny, g);
+    for (int side = 0; side < 4; side++) {
+        const int hi = side & 1;
+        if (p->bc[side] == HFO_BC_CUSTOM) {
+            if (p->hook[side]) ((hfo_bc2d_fn)p->hook[side])(w[0], w[1], w[2], w[3], g, nx, ny);
+        } else if (side < 2) {
+            for (int j = g; j < g + ny; j++) bc_side(hi, p->bc[side], w, 4, 1, (int64_t)j * tx, 1, nx, g);
+        } else {
+            for (int i = g; i < g + nx; i++) bc_side(hi, p->bc[side], w, 4, 2, i, tx, ny, g);
+        }
     }
 }
 

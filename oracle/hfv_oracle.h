@@ -31,7 +31,12 @@
  * boundary.h, system.h). */
 enum { HFO_MINMOD = 1, HFO_VAN_LEER = 2, HFO_MC = 3 };
 enum { HFO_EXACT = 1, HFO_HLLC = 2 };
-enum { HFO_BC_NONE = 1, HFO_BC_REFLECTIVE = 2, HFO_BC_TRANSMISSIVE = 3, HFO_BC_PERIODIC = 4 };
+enum { HFO_BC_NONE = 1, HFO_BC_REFLECTIVE = 2, HFO_BC_TRANSMISSIVE = 3, HFO_BC_PERIODIC = 4, HFO_BC_CUSTOM = 5 };
+
+/* custom boundary hooks, with the reference's signatures (boundary.h:17-33) */
+typedef void (*hfo_bc1d_fn)(double *rho, double *u, double *p, int num_ghost_cells_side, int num_cells_x);
+typedef void (*hfo_bc2d_fn)(double *rho, double *u, double *v, double *p, int num_ghost_cells_side,
+                            int num_cells_x, int num_cells_y);
 enum { HFO_CART1D = 1, HFO_CART2D = 2, HFO_CYL1D = 4, HFO_SPH1D = 5 };
 /* error codes (error.h:15-22) */
 enum { HFO_OK = 0, HFO_FAILURE = 1, HFO_VALUE_ERROR = 2 };
@@ -47,6 +52,7 @@ typedef struct hfo_params {
     double tol;
     int bc[4];          /* x_min, x_max, y_min, y_max */
     int nthreads;       /* OpenMP threads for the 2D sweeps (1 = serial) */
+    void *hook[4];      /* HFO_BC_CUSTOM sides: hfo_bc1d_fn / hfo_bc2d_fn */
 } hfo_params;
 
 typedef struct hfo_run {

@@ -69,9 +69,15 @@ def oracle_grid(sim, limiter, solver="riemann_solver_hllc", bcs=None, tol=1e-6, 
         names = [sim.bc.boundary_condition_x_min, sim.bc.boundary_condition_x_max,
                  sim.bc.boundary_condition_y_min, sim.bc.boundary_condition_y_max]
         bcs = tuple(n.decode() if n else None for n in names)
+    b = sim.bc
+    if sim.dim == 2:
+        hooks = (b.set_boundary_condition_cartesian_2d_x_min, b.set_boundary_condition_cartesian_2d_x_max,
+                 b.set_boundary_condition_cartesian_2d_y_min, b.set_boundary_condition_cartesian_2d_y_max)
+    else:
+        hooks = (b.set_boundary_condition_1d_x_min, b.set_boundary_condition_1d_x_max, None, None)
     grid = O.Grid(coord, sim.nx, sim.ny, sim.g, sysm.gamma, x=(sysm.x_min, sysm.x_max),
                   y=(sysm.y_min, sysm.y_max), bc=bcs, limiter=limiter, solver=solver, tol=tol,
-                  gravity=sysm.gravity, nthreads=nthreads)
+                  gravity=sysm.gravity, nthreads=nthreads, hooks=hooks)
     W, U = state(sim)
     grid.W[...] = W
     grid.U[...] = U
@@ -159,3 +165,92 @@ def assert_pow_special(out, x, y):
         assert np.all(np.abs(out[fin] - ref[fin]) <= np.spacing(np.abs(ref[fin])))
     special = ~fin
     np.testing.assert_array_equal(out[special], ref[special])
+
+
+# -- custom boundary hooks (boundary.h:17-33) ------------------------------
+def _plane(ptr, g, nx, ny):
+    tx, ty = nx + 2 * g, ny + 2 * g
+    return np.ctypeslib.as_array(ptr, shape=(ty * tx,)).reshape(ty, tx)
+
+
+def lid_hook(lid_u=1.0, lid_v=0.0):
+    """y_max hook of a lid-driven cavity: every ghost row above the top wall,
+    corner columns included, mirrors density and pressure of the rows below
+    and takes the lid's velocity.  The corner columns read x-ghost cells, so
+    the result depends on the x sides being filled first."""
+    def hook(rho, u, v, p, g, nx, ny):
+        R, U, V, P = (_plane(a, g, nx, ny) for a in (rho, u, v, p))
+        for j in range(g):
+            R[g + ny + j, :] = R[g + ny - 1 - j, :]
+            P[g + ny + j, :] = P[g + ny - 1 - j, :]
+            U[g + ny + j, :] = lid_u
+            V[g + ny + j, :] = lid_v
+    return hook
+
+
+def inflow_hook_2d(state=(1.2, 0.8, 0.1, 1.1)):
+    """x_min hook: a fixed inflow state in every ghost column of the
+    interior rows."""
+    def hook(rho, u, v, p, g, nx, ny):
+        for a, val in zip((rho, u, v, p), state):
+            _plane(a, g, nx, ny)[g:g + ny, :g] = val
+    return hook
+
+
+def copy_hook_2d(side):
+    """A hook that copies the nearest interior layer outwards (zero
+    gradient), for side 1 (x_max), 2 (y_min) or 3 (y_max)."""
+    def hook(rho, u, v, p, g, nx, ny):
+        for a in (rho, u, v, p):
+            A = _plane(a, g, nx, ny)
+            for i in range(g):
+                if side == 1:
+                    A[g:g + ny, g + nx + i] = A[g:g + ny, g + nx - 1]
+                elif side == 2:
+                    A[g - 1 - i, g:g + nx] = A[g, g:g + nx]
+                else:
+                    A[g + ny + i, g:g + nx] = A[g + ny - 1, g:g + nx]
+    return hook
+
+
+def setup_cavity(lib, n, g=2):
+    """Lid-driven cavity: reflective walls, a custom moving lid on top."""
+    s = driver.Simulation(lib, "cartesian_2d", 1.4, n, n, g)
+    s.set_boundary("reflective", "reflective", "reflective", "custom",
+                   custom={"set_boundary_condition_cartesian_2d_y_max": lid_hook()})
+    one = np.ones((n, n))
+    s.set_primitive([one, 0 * one, 0 * one, one / 1.4], interior_only=True)
+    s.apply_boundary()
+    s.primitive_to_conserved()
+    return s
+
+
+def setup_all_custom(lib, nx, ny, g, seed):
+    """Random interior state with every side filled by a hook."""
+    s = driver.Simulation(lib, "cartesian_2d", 1.4, nx, ny, g)
+    s.set_boundary("custom", "custom", "custom", "custom",
+                   custom={"set_boundary_condition_cartesian_2d_x_min": inflow_hook_2d(),
+                           "set_boundary_condition_cartesian_2d_x_max": copy_hook_2d(1),
+                           "set_boundary_condition_cartesian_2d_y_min": copy_hook_2d(2),
+                           "set_boundary_condition_cartesian_2d_y_max": copy_hook_2d(3)})
+    s.set_primitive(random_state_2d(nx, ny, seed), interior_only=True)
+    s.apply_boundary()
+    s.primitive_to_conserved()
+    return s
+
+
+def inflow_hook_1d(state=(1.5, 0.9, 1.3)):
+    def hook(rho, u, p, g, nx):
+        for a, val in zip((rho, u, p), state):
+            np.ctypeslib.as_array(a, shape=(nx + 2 * g,))[:g] = val
+    return hook
+
+
+def setup_inflow_1d(lib, n, g=2):
+    """Sod tube whose left ghost cells hold a fixed inflow state (custom)."""
+    s = driver.Simulation(lib, "cartesian_1d", 1.4, n, g=g)
+    s.set_primitive(workloads.sod(n, g))
+    s.set_boundary("custom", "transmissive", custom={"set_boundary_condition_1d_x_min": inflow_hook_1d()})
+    s.apply_boundary()
+    s.primitive_to_conserved()
+    return s

@@ -19,7 +19,7 @@ dp = C.POINTER(C.c_double)
 COORD = {"cartesian_1d": 1, "cartesian_2d": 2, "cylindrical_1d": 4, "spherical_1d": 5}
 LIMITER = {"minmod": 1, "van_leer": 2, "monotonized_central": 3}
 SOLVER = {"riemann_solver_exact": 1, "riemann_solver_hllc": 2}
-BC = {"none": 1, "reflective": 2, "transmissive": 3, "periodic": 4}
+BC = {"none": 1, "reflective": 2, "transmissive": 3, "periodic": 4, "custom": 5}
 
 
 class Params(C.Structure):
@@ -28,6 +28,7 @@ class Params(C.Structure):
         ("gamma", C.c_double), ("gravity", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
         ("limiter", C.c_int), ("solver", C.c_int), ("tol", C.c_double),
         ("bc", C.c_int * 4), ("nthreads", C.c_int),
+        ("hook", C.c_void_p * 4),
     ]
 
 
@@ -116,6 +117,7 @@ class Grid:
     tol: float = 1e-6
     gravity: float = 0.0
     nthreads: int = 1
+    hooks: tuple = (None, None, None, None)  # ctypes callbacks of the custom sides
     W: np.ndarray = field(default=None, repr=False)
     U: np.ndarray = field(default=None, repr=False)
 
@@ -139,6 +141,8 @@ class Grid:
         for k in range(4):
             self.p.bc[k] = BC[self.bc[k]] if k < len(self.bc) and self.bc[k] else 0
         self.p.nthreads = self.nthreads
+        for k, h in enumerate(self.hooks):
+            self.p.hook[k] = C.cast(h, C.c_void_p).value if h else None
         self.mid_x = np.zeros(self.tx)
         self.mid_y = np.zeros(max(self.ty, 1))
         self.area_x = np.zeros(self.tx + 1)
