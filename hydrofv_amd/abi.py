@@ -192,6 +192,7 @@ PROTOTYPES = {
     "muscl_hancock_2d": (ErrorStatus, _SEVEN),
     "godunov_first_order_1d": (ErrorStatus, _SEVEN),
     "godunov_first_order_2d": (ErrorStatus, _SEVEN),
+    "random_choice_1d": (ErrorStatus, _SEVEN),
     "store_snapshot": (ErrorStatus, [_P(BoundaryConditionParam), _P(System), _P(IntegratorParam),
                                      _P(SimulationStatus), _P(StoringParam)]),
 }

@@ -73,6 +73,37 @@ __device__ void c2p_all(const Grid1D& G, const Arrays1D& A, int t)
     }
 }
 
+// system.c:849-917 (1D): conserved state from the primitive one, every cell
+__device__ void p2c_all(const Grid1D& G, const Arrays1D& A, int t)
+{
+    for (int k = t; k < G.tx; k += T1D) {
+        const double r = A.w[0][k], vx = A.w[1][k], pr = A.w[2][k], vol = A.vol[k];
+        const double m = r * vol;
+        A.u[0][k] = m;
+        A.u[1][k] = m * vx;
+        A.u[2][k] = vol * (0.5 * r * vx * vx + pr / (G.gamma - 1.0));
+    }
+}
+
+// In place of a limiter: Glimm's random choice method (random_choice_1d,
+// integrator_random_choice_1d.c), which samples exact Riemann solutions.
+constexpr int RANDOM_CHOICE = -1;
+
+// integrator_random_choice_1d.c:28-43: van der Corput sequence, k1 = 5, k2 = 3
+__device__ __forceinline__ double van_der_corput(long long n)
+{
+    double result = 0.0;
+    const int base = 5;
+    double bk = 1.0 / (double)base;
+    while (n > 0) {
+        const int a = (int)(n % base);
+        result += ((3 * a) % base) * bk;
+        n /= base;
+        bk /= (double)base;
+    }
+    return result;
+}
+
 // source_term.c:28-43
 __device__ __forceinline__ void geo_rate(double pre, double gamma, double rho, double sm, double se, double* o)
 {
@@ -88,17 +119,20 @@ __device__ __forceinline__ void geo_rate(double pre, double gamma, double rho, d
 // 128-231): no reconstruction, and a failed flux ends the run with its error
 // (the serial loop's goto at :172-176), reported for the lowest failing
 // interface; the update, c2p, boundary fill and geometric source are those
-// of the MUSCL-Hancock step.
+// of the MUSCL-Hancock step.  LIM == RANDOM_CHOICE: random_choice_1d, which
+// samples W directly and then fills the boundary, converts to conserved
+// variables and adds the geometric source.
 template <int LIM, int SOLVER>
 __global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays1D A, Ctrl* ctrl, long long budget)
 {
     __shared__ double red[T1D / 64];
-    __shared__ int fail_i, fail_kind;
+    __shared__ int fail_i, fail_last, fail_kind;
     const int t = threadIdx.x;
     const int g = G.g, n = G.n;
     const double gamma = G.gamma;
     if (t == 0) {
         fail_i = INT_MAX;
+        fail_last = -1;
         fail_kind = ERR_NONE;
     }
     __syncthreads();
@@ -130,7 +164,57 @@ __global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays
             break;
         }
         const double dt = plan.dt;
-        if constexpr (LIM == FIRST_ORDER) {
+        if constexpr (LIM == RANDOM_CHOICE) {
+            // integrator_random_choice_1d.c:160-231: a copy of W (in the lo
+            // arrays), then every cell g..g+n samples the exact solution of
+            // the problem with its left (theta <= 1/2) or right neighbour
+            const double theta = van_der_corput(plan.n + 1);
+            for (int k = t; k < G.tx; k += T1D)
+                for (int f = 0; f < 3; f++) A.lo[f][k] = A.w[f][k];
+            __syncthreads();
+            int my_i = -1, my_kind = ERR_NONE;
+            for (int i = g + t; i <= g + n; i += T1D) {
+                int a, b;
+                double speed;
+                if (theta <= 0.5) {
+                    a = i - 1;
+                    b = i;
+                    speed = theta * rp.dx / dt;
+                } else {
+                    // with one ghost layer the reference reads one cell past
+                    // its arrays here; the read is clamped to the last cell
+                    a = i;
+                    b = min(i + 1, G.tx - 1);
+                    speed = (theta - 1.0) * rp.dx / dt;
+                }
+                const Sample S = exact_sample(gamma, G.tol, A.lo[0][a], A.lo[1][a], 0.0, A.lo[2][a], A.lo[0][b],
+                                              A.lo[1][b], 0.0, A.lo[2][b], speed);
+                A.w[0][i] = S.w.r;
+                A.w[1][i] = S.w.u;
+                A.w[2][i] = S.w.p;
+                if (S.rc != ERR_NONE) {
+                    my_i = i;  // the last failure of the serial loop is reported
+                    my_kind = S.rc;
+                }
+            }
+            if (my_i >= 0) atomicMax(&fail_last, my_i);
+            __syncthreads();
+            if (my_i >= 0 && my_i == fail_last) fail_kind = my_kind;
+            __syncthreads();
+            if (fail_last >= 0) {
+                c.err_kind = fail_kind;
+                c.err_cell = fail_last;
+                c.t_prev = c.t;
+                c.t = plan.t + dt;
+                c.dt = dt;
+                c.num_steps = plan.n + 1;
+                break;
+            }
+            bc_1d(G, A.w, t);
+            __syncthreads();
+            p2c_all(G, A, t);
+            __syncthreads();
+        } else if constexpr (LIM == FIRST_ORDER) {
             // interface fluxes 0..n from the cells' own states (:153-171)
             int my_i = INT_MAX, my_kind = ERR_NONE;
             for (int i = t; i <= n; i += T1D) {
@@ -188,24 +272,26 @@ __global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays
             for (int q = 0; q < 3; q++) A.fl[q][i] = f[q] * dt;
         }
         }
-        __syncthreads();
-        // update in the reference's order: a cell first gains its left face's
-        // flux, then loses its right face's, both scaled by its own face-array
-        // entry (287-294)
-        for (int k = g - 1 + t; k <= g + n; k += T1D) {
-            const double ak = A.area[k];
-            for (int q = 0; q < 3; q++) {
-                double x = A.u[q][k];
-                if (k >= g) x += A.fl[q][k - g] * ak;
-                if (k < g + n) x -= A.fl[q][k - g + 1] * ak;
-                A.u[q][k] = x;
+        if constexpr (LIM != RANDOM_CHOICE) {
+            __syncthreads();
+            // update in the reference's order: a cell first gains its left
+            // face's flux, then loses its right face's, both scaled by its own
+            // face-array entry (287-294)
+            for (int k = g - 1 + t; k <= g + n; k += T1D) {
+                const double ak = A.area[k];
+                for (int q = 0; q < 3; q++) {
+                    double x = A.u[q][k];
+                    if (k >= g) x += A.fl[q][k - g] * ak;
+                    if (k < g + n) x -= A.fl[q][k - g + 1] * ak;
+                    A.u[q][k] = x;
+                }
             }
+            __syncthreads();
+            c2p_all(G, A, t);
+            __syncthreads();
+            bc_1d(G, A.w, t);
+            __syncthreads();
         }
-        __syncthreads();
-        c2p_all(G, A, t);
-        __syncthreads();
-        bc_1d(G, A.w, t);
-        __syncthreads();
         if (G.coord == 4 || G.coord == 5) {
             // source_term.c:45-175: RK4 per interior cell, then c2p and fill
             const double alpha = (G.coord == 4) ? 1.0 : 2.0;
@@ -358,10 +444,12 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
 {
     hipStream_t s = (hipStream_t)stream;
     if (cfg->first_order != 0 && cfg->first_order != 1) return HFV_EINVAL;
+    if (cfg->random_choice != 0 && cfg->random_choice != 1) return HFV_EINVAL;
     if (cfg->steps_start < 0 || !(cfg->t_start >= 0.0) || cfg->t_pause < 0.0) return HFV_EINVAL;
-    if (cfg->num_cells_x < 2 || cfg->num_ghost_cells_side < (cfg->first_order ? 1 : 2)) return HFV_EINVAL;
-    if (!cfg->first_order && (cfg->slope_limiter < 1 || cfg->slope_limiter > 3)) return HFV_EINVAL;
-    if (cfg->riemann_solver != HLLC && cfg->riemann_solver != EXACT) return HFV_EINVAL;
+    const bool order1 = cfg->first_order || cfg->random_choice;  // one ghost layer suffices
+    if (cfg->num_cells_x < 2 || cfg->num_ghost_cells_side < (order1 ? 1 : 2)) return HFV_EINVAL;
+    if (!order1 && (cfg->slope_limiter < 1 || cfg->slope_limiter > 3)) return HFV_EINVAL;
+    if (!cfg->random_choice && cfg->riemann_solver != HLLC && cfg->riemann_solver != EXACT) return HFV_EINVAL;
     for (int q = 0; q < 2; q++)
         // BC_CUSTOM: the device leaves those ghosts as c2p of their own
         // state; the host applies the hook between steps (hfv_integrator.c)
@@ -428,7 +516,9 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
     Ctrl c;
     for (;;) {
 #define HFV_L1D(L, S) hipLaunchKernelGGL((k_mh1d_run<L, S>), dim3(1), dim3(T1D), 0, s, G, rp, A, dctrl.p, budget)
-        if (cfg->first_order) {
+        if (cfg->random_choice) {
+            HFV_L1D(RANDOM_CHOICE, EXACT);
+        } else if (cfg->first_order) {
             if (hllc) HFV_L1D(FIRST_ORDER, HLLC);
             else HFV_L1D(FIRST_ORDER, EXACT);
         } else if (hllc) {

@@ -1,7 +1,7 @@
 /*
  * hfv_integrator.c -- muscl_hancock_1d / muscl_hancock_2d (integrator.h:145-176)
- * and godunov_first_order_1d / godunov_first_order_2d (integrator.h:99-130)
- * running on the MI355X.
+ * godunov_first_order_1d / godunov_first_order_2d (integrator.h:99-130) and
+ * random_choice_1d (integrator.h:76-84) running on the MI355X.
  *
  * The reference's time loop (integrator_muscl_hancock.c:124-349, 530-1160)
  * becomes: copy the System's arrays to the device once, run the steps there
@@ -249,9 +249,10 @@ static ErrorStatus run_2d(BoundaryConditionParam *bc, System *system, Integrator
     return make_success_error_status();
 }
 
+/* scheme: 0 MUSCL-Hancock, 1 first-order Godunov, 2 random choice */
 static ErrorStatus run_1d(BoundaryConditionParam *bc, System *system, IntegratorParam *ip, StoringParam *sp,
                           SimulationParam *prm, SimulationStatus *status, long long max_steps, double fixed_dt,
-                          int first_order)
+                          int scheme)
 {
     const int f = system->coord_sys_flag_;
     if (f != COORD_SYS_CARTESIAN_1D && f != COORD_SYS_CYLINDRICAL_1D && f != COORD_SYS_SPHERICAL_1D) {
@@ -283,7 +284,14 @@ static ErrorStatus run_1d(BoundaryConditionParam *bc, System *system, Integrator
     cfg.tf = prm->tf;
     cfg.max_steps = max_steps;
     cfg.fixed_dt = fixed_dt;
-    cfg.first_order = first_order;
+    cfg.first_order = scheme == 1;
+    cfg.random_choice = scheme == 2;
+    if (scheme == 2 && has_custom_bc(bc, 0)) {
+        /* the hook would have to run between the sampling and the conversion
+         * to conserved variables, inside the device step */
+        return WRAP_RAISE_ERROR(NOT_IMPLEMENTED_ERROR,
+                                "Custom boundary conditions with the random-choice integrator are not provided by this library.");
+    }
     status->num_steps = 0;
     status->dt = 0.0;
     status->t = 0.0;
@@ -358,6 +366,20 @@ ErrorStatus hfv_godunov_first_order_1d_run(BoundaryConditionParam *bc, System *s
 {
     (void)settings;
     return run_1d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 1);
+}
+
+ErrorStatus hfv_random_choice_1d_run(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
+                                     StoringParam *sp, Settings *settings, SimulationParam *prm,
+                                     SimulationStatus *status, long long max_steps, double fixed_dt)
+{
+    (void)settings;
+    return run_1d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 2);
+}
+
+ErrorStatus random_choice_1d(BoundaryConditionParam *bc, System *system, IntegratorParam *ip, StoringParam *sp,
+                             Settings *settings, SimulationParam *prm, SimulationStatus *status)
+{
+    return hfv_random_choice_1d_run(bc, system, ip, sp, settings, prm, status, -1, 0.0);
 }
 
 ErrorStatus muscl_hancock_2d(BoundaryConditionParam *bc, System *system, IntegratorParam *ip, StoringParam *sp,

@@ -214,8 +214,7 @@ ErrorStatus integrator_launch_simulation(BoundaryConditionParam *bc, System *sys
     case INTEGRATOR_GODUNOV_FIRST_ORDER_2D:
         return WRAP_TRACEBACK(godunov_first_order_2d(bc, system, ip, sp, st, prm, status));
     case INTEGRATOR_RANDOM_CHOICE_1D:
-        return WRAP_RAISE_ERROR(NOT_IMPLEMENTED_ERROR,
-                                "The random-choice integrator is not provided by this library.");
+        return WRAP_TRACEBACK(random_choice_1d(bc, system, ip, sp, st, prm, status));
     default:
         return WRAP_RAISE_ERROR(VALUE_ERROR, "Integrator flag not recognized.");
     }

@@ -157,7 +157,7 @@ class Simulation:
                store_initial=False) -> RunResult:
         """Run the integrator.  ``max_steps`` / ``fixed_dt`` select the
         library's step-limited / constant-dt loop (hfv_muscl_hancock_*_run,
-        hfv_godunov_first_order_*_run).  ``store_dir`` (ending in "/") turns
+        hfv_godunov_first_order_*_run, hfv_random_choice_1d_run).  ``store_dir`` (ending in "/") turns
         on snapshots every ``store_interval`` of simulation time."""
         ip = self.params(integrator, riemann_solver, slope_limiter, cfl, tol, shrink_factor, shrink_steps)
         sp = self.lib.get_new_storing_param()
@@ -171,7 +171,8 @@ class Simulation:
         prm = abi.SimulationParam(tf=tf)
         status = abi.SimulationStatus()
         if max_steps is not None or fixed_dt is not None:
-            scheme = "godunov_first_order" if integrator.startswith("godunov_first_order") else "muscl_hancock"
+            scheme = next((k for k in ("godunov_first_order", "random_choice") if integrator.startswith(k)),
+                          "muscl_hancock")
             entry = f"hfv_{scheme}_{self.dim}d_run"
         if entry != "launch_simulation":
             # direct integrator call: finalize the parameters the way

@@ -51,6 +51,10 @@ class MH1DConfig(C.Structure):
         ("max_steps", C.c_longlong),
         ("fixed_dt", C.c_double),
         ("first_order", C.c_int),
+        ("random_choice", C.c_int),
+        ("t_start", C.c_double),
+        ("steps_start", C.c_longlong),
+        ("t_pause", C.c_double),
     ]
 
 
@@ -87,6 +91,7 @@ PROTOTYPES = {
     "hfv_muscl_hancock_2d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
     "hfv_godunov_first_order_1d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
     "hfv_godunov_first_order_2d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
+    "hfv_random_choice_1d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
 }
 
 

@@ -147,6 +147,10 @@ typedef struct hfv_mh1d_config {
     int first_order;               /* 1: first-order Godunov (godunov_first_order_1d,
                                       integrator_godunov_first_order.c:26-268), which
                                       reports a failed flux; 0: MUSCL-Hancock */
+    int random_choice;             /* 1: random_choice_1d (integrator_random_choice_1d.c):
+                                      Glimm's method, exact Riemann sampling at van der
+                                      Corput positions; riemann_solver and slope_limiter
+                                      ignored, 1 ghost layer suffices */
     /* Resuming and pausing, for snapshots: the run starts at time t_start
      * after steps_start steps (0, 0: a fresh run; W must then be the state
      * the run left), and with t_pause > 0 it stops after the first step

@@ -5,9 +5,9 @@
  * pulls in): the same type names, field order and layouts, flag values,
  * function names and signatures, so a driver written against the reference
  * (e.g. examples/kelvin_helmholtz_2d/kelvin_helmholtz_2d.c) compiles and links
- * against this library unchanged.  muscl_hancock_1d / muscl_hancock_2d and
- * godunov_first_order_1d / godunov_first_order_2d run on the MI355X; the
- * set-up helpers (system_init, conversions, boundary fill of
+ * against this library unchanged.  muscl_hancock_1d / muscl_hancock_2d,
+ * godunov_first_order_1d / godunov_first_order_2d and random_choice_1d run
+ * on the MI355X; the set-up helpers (system_init, conversions, boundary fill of
  * the initial state) operate on the host arrays of System as in the reference.
  *
  * Each declaration cites the reference interface it replaces.
@@ -241,15 +241,19 @@ ErrorStatus muscl_hancock_2d(HFV_INTEGRATOR_ARGS);
 /* First-order Godunov on the GPU: integrator.h:99-107 (1D), 122-130 (2D). */
 ErrorStatus godunov_first_order_1d(HFV_INTEGRATOR_ARGS);
 ErrorStatus godunov_first_order_2d(HFV_INTEGRATOR_ARGS);
+/* Random choice (Glimm) on the GPU: integrator.h:76-84. */
+ErrorStatus random_choice_1d(HFV_INTEGRATOR_ARGS);
 
 /* Extension (no reference counterpart): the same loops with a step limit
  * (max_steps >= 0) and/or a constant dt (fixed_dt > 0); muscl_hancock_{1d,2d}
- * and godunov_first_order_{1d,2d} call these with (-1, 0.0).  Used for fixed-length benchmark runs and for
+ * godunov_first_order_{1d,2d} and random_choice_1d call these with
+ * (-1, 0.0).  Used for fixed-length benchmark runs and for
  * fixed-dt forward solves. */
 ErrorStatus hfv_muscl_hancock_1d_run(HFV_INTEGRATOR_ARGS, long long max_steps, double fixed_dt);
 ErrorStatus hfv_muscl_hancock_2d_run(HFV_INTEGRATOR_ARGS, long long max_steps, double fixed_dt);
 ErrorStatus hfv_godunov_first_order_1d_run(HFV_INTEGRATOR_ARGS, long long max_steps, double fixed_dt);
 ErrorStatus hfv_godunov_first_order_2d_run(HFV_INTEGRATOR_ARGS, long long max_steps, double fixed_dt);
+ErrorStatus hfv_random_choice_1d_run(HFV_INTEGRATOR_ARGS, long long max_steps, double fixed_dt);
 
 /* ---- storing.h:21-49 --------------------------------------------------- */
 StoringParam get_new_storing_param(void);                            /* storing.h:21 */

@@ -1,6 +1,6 @@
 /*
  * hfv_oracle.c -- CPU oracle for the MUSCL-Hancock finite-volume path (and
- * the first-order Godunov scheme that shares its fluxes).
+ * the first-order Godunov and random-choice schemes that share its parts).
  *
  * TEST INFRASTRUCTURE ONLY (see hfv_oracle.h).  Plain C restatement of the
  * reference algorithm; each function cites the reference file:line whose
@@ -9,7 +9,8 @@
  * Reference files (hydroFV, src/):
  *   slope_limiter.c, utils.c, riemann_solver.c, riemann_solver_hllc.c,
  *   riemann_solver_exact.c, system.c, boundary.c, source_term.c,
- *   integrator_muscl_hancock.c, integrator_godunov_first_order.c
+ *   integrator_muscl_hancock.c, integrator_godunov_first_order.c,
+ *   integrator_random_choice_1d.c
  */
 #include "hfv_oracle.h"
 
@@ -1184,10 +1185,81 @@ int hfo_god1d_step(const hfo_params *p, double dt, const double *area_x, const d
 }
 
 /* ======================================================================== */
-/* Time loops: integrator_muscl_hancock.c:124-349 (1D), 530-1160 (2D);      */
-/* integrator_godunov_first_order.c:128-240 (1D), 363-546 (2D), the same.   */
+/* Random choice (Glimm) 1D: integrator_random_choice_1d.c                  */
 /* ======================================================================== */
-static void run_loop(const hfo_params *p, hfo_run *r, int dim, int first_order, const double *area_x,
+/* 28-43: the van der Corput sequence with k1 = 5, k2 = 3 (lines 25-26) */
+double hfo_van_der_corput(int64_t n)
+{
+    double result = 0.0;
+    const int base = 5;
+    double bk = 1.0 / (double)base;
+    while (n > 0) {
+        const int a = (int)(n % base);
+        result += ((3 * a) % base) * bk;
+        n /= base;
+        bk /= (double)base;
+    }
+    return result;
+}
+
+/* 160-242: every cell g..g+n takes the exact solution sampled at
+ * theta*dx/dt of the problem with its left neighbour (theta <= 1/2) or at
+ * (theta-1)*dx/dt with its right neighbour, from a copy of W; then the
+ * boundary fill, p2c and the geometric source.  A failed solve does not stop
+ * the loop; the last one is reported (213-221).  With one ghost layer the
+ * reference reads one cell past its arrays when theta > 1/2; the read is
+ * clamped to the last cell here. */
+int hfo_rc1d_step(const hfo_params *p, double dt, int64_t step, const double *mid_x, const double *volume,
+                  double *const *w, double *const *u, int64_t *err_cell)
+{
+    const int g = p->g, n = p->nx, tx = n + 2 * g;
+    const double theta = hfo_van_der_corput(step + 1);
+    double *tmp = malloc((size_t)3 * tx * sizeof(double));
+    for (int f = 0; f < 3; f++) memcpy(tmp + (size_t)f * tx, w[f], (size_t)tx * sizeof(double));
+    int err = HFO_OK;
+    for (int i = g; i < g + n + 1; i++) {
+        int a, b;
+        double speed;
+        if (theta <= 0.5) {
+            a = i - 1;
+            b = i;
+            speed = theta * p->dx / dt;
+        } else {
+            a = i;
+            b = (i + 1 < tx) ? i + 1 : tx - 1;
+            speed = (theta - 1.0) * p->dx / dt;
+        }
+        const double wl[3] = {tmp[a], tmp[tx + a], tmp[2 * tx + a]};
+        const double wr[3] = {tmp[b], tmp[tx + b], tmp[2 * tx + b]};
+        double sol[3];
+        const int rc = exact_sample(p->gamma, p->tol, wl, wr, speed, sol, NULL);
+        w[0][i] = sol[0];
+        w[1][i] = sol[1];
+        w[2][i] = sol[2];
+        if (rc != HFO_OK) {
+            err = rc;
+            if (err_cell) *err_cell = i;
+        }
+    }
+    free(tmp);
+    if (err != HFO_OK) return err;
+    hfo_bc(p, w);
+    hfo_p2c(p, volume, w, u);
+    if (p->coord == HFO_CYL1D || p->coord == HFO_SPH1D) {
+        hfo_geometry_source_1d(p, dt, mid_x, volume, w, u);
+        hfo_c2p(p, volume, u, w);
+        hfo_bc(p, w);
+    }
+    return HFO_OK;
+}
+
+/* ======================================================================== */
+/* Time loops: integrator_muscl_hancock.c:124-349 (1D), 530-1160 (2D);      */
+/* integrator_godunov_first_order.c:128-240 (1D), 363-546 (2D), and         */
+/* integrator_random_choice_1d.c:138-254, the same loop.                    */
+/* scheme: 0 MUSCL-Hancock, 1 first-order Godunov, 2 random choice (1D).    */
+/* ======================================================================== */
+static void run_loop(const hfo_params *p, hfo_run *r, int dim, int scheme, const double *area_x,
                      const double *area_y, const double *mid_x, const double *volume, double *const *w,
                      double *const *u)
 {
@@ -1212,11 +1284,13 @@ static void run_loop(const hfo_params *p, hfo_run *r, int dim, int first_order, 
         }
         int rc;
         if (dim == 2) {
-            rc = first_order ? hfo_god2d_step(p, r->dt, area_x, area_y, volume, w, u, &r->error_cell)
-                             : hfo_mh2d_step(p, r->dt, area_x, area_y, volume, w, u, &r->error_cell);
+            rc = scheme ? hfo_god2d_step(p, r->dt, area_x, area_y, volume, w, u, &r->error_cell)
+                        : hfo_mh2d_step(p, r->dt, area_x, area_y, volume, w, u, &r->error_cell);
+        } else if (scheme == 2) {
+            rc = hfo_rc1d_step(p, r->dt, r->num_steps, mid_x, volume, w, u, &r->error_cell);
         } else {
-            rc = first_order ? hfo_god1d_step(p, r->dt, area_x, mid_x, volume, w, u, &r->error_cell)
-                             : hfo_mh1d_step(p, r->dt, area_x, mid_x, volume, w, u, &r->error_cell);
+            rc = scheme ? hfo_god1d_step(p, r->dt, area_x, mid_x, volume, w, u, &r->error_cell)
+                        : hfo_mh1d_step(p, r->dt, area_x, mid_x, volume, w, u, &r->error_cell);
         }
         if (rc != HFO_OK) {
             r->error = rc;
@@ -1249,4 +1323,10 @@ void hfo_god1d_run(const hfo_params *p, hfo_run *r, const double *area_x, const 
                    const double *volume, double *const *w, double *const *u)
 {
     run_loop(p, r, 1, 1, area_x, NULL, mid_x, volume, w, u);
+}
+
+void hfo_rc1d_run(const hfo_params *p, hfo_run *r, const double *area_x, const double *mid_x,
+                  const double *volume, double *const *w, double *const *u)
+{
+    run_loop(p, r, 1, 2, area_x, NULL, mid_x, volume, w, u);
 }

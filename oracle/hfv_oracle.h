@@ -16,7 +16,8 @@
  *     hfo_vacuum_1d/2d, hfo_exact_1d, hfo_flux_exact_1d, hfo_geometry,
  *     hfo_p2c, hfo_c2p, hfo_bc, hfo_dt_1d/2d, hfo_gravity_2d,
  *     hfo_geometry_source_1d, hfo_mh1d_run, hfo_mh2d_run (HLLC),
- *     hfo_god1d_run (HLLC, exact), hfo_god2d_run (HLLC)
+ *     hfo_god1d_run (HLLC, exact), hfo_god2d_run (HLLC), hfo_van_der_corput,
+ *     hfo_rc1d_run
  *   pinned by composition and closed forms only (no reference counterpart):
  *     hfo_flux_exact_2d (normal components equal hfo_flux_exact_1d bitwise;
  *     transverse momentum flux = mass flux x upwinded transverse velocity),
@@ -130,5 +131,13 @@ void hfo_god2d_run(const hfo_params *p, hfo_run *r, const double *area_x, const 
                    const double *volume, double *const *w, double *const *u);
 void hfo_god1d_run(const hfo_params *p, hfo_run *r, const double *area_x, const double *mid_x,
                    const double *volume, double *const *w, double *const *u);
+
+/* random_choice_1d (integrator_random_choice_1d.c): the van der Corput
+ * sequence, one step (step = steps taken before it), the time loop. */
+double hfo_van_der_corput(int64_t n);
+int hfo_rc1d_step(const hfo_params *p, double dt, int64_t step, const double *mid_x, const double *volume,
+                  double *const *w, double *const *u, int64_t *err_cell);
+void hfo_rc1d_run(const hfo_params *p, hfo_run *r, const double *area_x, const double *mid_x,
+                  const double *volume, double *const *w, double *const *u);
 
 #endif

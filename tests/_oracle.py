@@ -78,6 +78,9 @@ def lib():
         L.hfo_mh1d_run.argtypes = [C.POINTER(Params), C.POINTER(Run), dp, dp, dp, C.POINTER(dp), C.POINTER(dp)]
         L.hfo_god2d_run.argtypes = [C.POINTER(Params), C.POINTER(Run), dp, dp, dp, C.POINTER(dp), C.POINTER(dp)]
         L.hfo_god1d_run.argtypes = [C.POINTER(Params), C.POINTER(Run), dp, dp, dp, C.POINTER(dp), C.POINTER(dp)]
+        L.hfo_rc1d_run.argtypes = [C.POINTER(Params), C.POINTER(Run), dp, dp, dp, C.POINTER(dp), C.POINTER(dp)]
+        L.hfo_van_der_corput.restype = C.c_double
+        L.hfo_van_der_corput.argtypes = [C.c_int64]
         L.hfo_mh2d_step.restype = C.c_int
         L.hfo_mh2d_step.argtypes = [C.POINTER(Params), C.c_double, dp, dp, dp, C.POINTER(dp), C.POINTER(dp),
                                     C.POINTER(C.c_int64)]
@@ -178,9 +181,10 @@ class Grid:
         lib().hfo_bc(C.byref(self.p), _ptrs([f.reshape(-1) for f in self.W]))
 
     def run(self, cfl, tf=np.inf, max_steps=-1, fixed_dt=0.0, shrink_factor=0.2, shrink_steps=10,
-            first_order=False):
+            first_order=False, random_choice=False):
         """The MUSCL-Hancock loop, or with first_order the first-order
-        Godunov loop (godunov_first_order_{1d,2d})."""
+        Godunov loop (godunov_first_order_{1d,2d}), or with random_choice
+        random_choice_1d."""
         r = Run(cfl=cfl, shrink_factor=shrink_factor, shrink_steps=shrink_steps, tf=tf,
                 max_steps=max_steps, fixed_dt=fixed_dt)
         W = _ptrs([f.reshape(-1) for f in self.W])
@@ -189,7 +193,7 @@ class Grid:
             fn = lib().hfo_god2d_run if first_order else lib().hfo_mh2d_run
             fn(C.byref(self.p), C.byref(r), _ptr(self.area_x), _ptr(self.area_y), _ptr(self.volume), W, U)
         else:
-            fn = lib().hfo_god1d_run if first_order else lib().hfo_mh1d_run
+            fn = lib().hfo_rc1d_run if random_choice else lib().hfo_god1d_run if first_order else lib().hfo_mh1d_run
             fn(C.byref(self.p), C.byref(r), _ptr(self.area_x), _ptr(self.mid_x), _ptr(self.volume), W, U)
         return r
 

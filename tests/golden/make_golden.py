@@ -12,6 +12,7 @@ reference's own C sources) is driven through its public API only.  Outputs:
   runs_2d.npz         initial and final states of full 2D runs
   runs_1d.npz         the same for 1D runs (Sod, curvilinear, boundaries)
   runs_godunov.npz    the same for godunov_first_order_{1d,2d} runs
+  runs_random_choice.npz  the same for random_choice_1d runs
   sod_exact_t0.2.npz  copy of examples/sod_shock_1d/sol_cartesian_1d_0.2.npz,
                       the reference's exact Sod solution at t = 0.2 (data)
 """
@@ -161,16 +162,44 @@ def runs_godunov():
     return out
 
 
+def runs_random_choice():
+    """random_choice_1d runs (exact solver sampling; CFL 0.5): Sod in the
+    three geometries and a periodic tube, to t = 0.2 / 0.05."""
+    out = {}
+    ref = abi.bind(C.CDLL(O.REF_SO))
+    cases = {
+        "rc_sod200": (lambda L: K.setup_sod(L, 200), 0.5, 0.2, 1e-6),
+        "rc_sod100_tol8": (lambda L: K.setup_sod(L, 100), 0.5, 0.2, 1e-8),
+        "rc_cyl64": (lambda L: K.setup_sod(L, 64, coord="cylindrical_1d", bc=("reflective", "transmissive")),
+                     0.5, 0.2, 1e-6),
+        "rc_sph64": (lambda L: K.setup_sod(L, 64, coord="spherical_1d", bc=("reflective", "transmissive")),
+                     0.5, 0.2, 1e-6),
+        "rc_sod50g3_periodic": (lambda L: K.setup_sod(L, 50, g=3, bc=("periodic", "periodic")), 0.5, 0.05, 1e-6),
+    }
+    for key, (mk, cfl, tf, tol) in cases.items():
+        sim = mk(ref)
+        W0, U0 = K.state(sim)
+        res = sim.launch("random_choice_1d", "riemann_solver_exact", "minmod", cfl, tf, tol=tol)
+        W1, U1 = K.state(sim)
+        for k, v in dict(W0=W0, U0=U0, W1=W1, U1=U1, t=res.t, dt=res.dt, n=res.num_steps).items():
+            out[f"{key}/{k}"] = v
+    return out
+
+
 def main():
     if not O.ref_available():
         sys.exit("build the reference first: make -C oracle ref")
     if sys.argv[1:] == ["godunov"]:
         np.savez_compressed(os.path.join(HERE, "runs_godunov.npz"), **runs_godunov())
         return
+    if sys.argv[1:] == ["random_choice"]:
+        np.savez_compressed(os.path.join(HERE, "runs_random_choice.npz"), **runs_random_choice())
+        return
     np.savez_compressed(os.path.join(HERE, "scalar_vectors.npz"), **scalar_vectors())
     np.savez_compressed(os.path.join(HERE, "runs_2d.npz"), **runs_2d())
     np.savez_compressed(os.path.join(HERE, "runs_1d.npz"), **runs_1d())
     np.savez_compressed(os.path.join(HERE, "runs_godunov.npz"), **runs_godunov())
+    np.savez_compressed(os.path.join(HERE, "runs_random_choice.npz"), **runs_random_choice())
     src = os.path.join(REF_EXAMPLES, "sod_shock_1d", "sol_cartesian_1d_0.2.npz")
     if os.path.exists(src):
         shutil.copyfile(src, os.path.join(HERE, "sod_exact_t0.2.npz"))

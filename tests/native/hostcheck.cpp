@@ -5,6 +5,9 @@
 // Test infrastructure only; built by tests/native/Makefile.
 #include "../../hydrofv_amd/csrc/device/hfv_adjoint.hpp"
 #include "../../hydrofv_amd/csrc/device/hfv_pow.hpp"
+#include "hydrofv/hfv_gpu.h"
+
+#include <cstddef>
 
 using namespace hfv;
 
@@ -22,6 +25,21 @@ void limits(long long n, const double* a, const double* b, double* d, double* da
 }  // namespace
 
 extern "C" {
+
+// sizes and selected field offsets of the structs of hfv_gpu.h, for the
+// ctypes mirrors: out[0..3] sizeof(mh2d, mh1d, run_status, adj2d config),
+// out[4..] offsets of the fields added last to each config
+void hc_struct_layout(long long* out)
+{
+    out[0] = sizeof(hfv_mh2d_config);
+    out[1] = sizeof(hfv_mh1d_config);
+    out[2] = sizeof(hfv_run_status);
+    out[3] = sizeof(hfv_adj2d_config);
+    out[4] = offsetof(hfv_mh2d_config, first_order);
+    out[5] = offsetof(hfv_mh1d_config, t_pause);
+    out[6] = offsetof(hfv_mh1d_config, random_choice);
+    out[7] = offsetof(hfv_adj2d_config, slope_limiter);
+}
 
 // out[n] = pow_pos(x[n], y[n]) (the kernels' pow of the two-rarefaction guess)
 void hc_pow_pos(long long n, const double* x, const double* y, double* out)

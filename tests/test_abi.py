@@ -29,6 +29,7 @@ def declared_functions(path):
 def test_headers_declare_the_hot_path():
     names = set().union(*(declared_functions(h) for h in HEADERS))
     for required in ("muscl_hancock_2d", "muscl_hancock_1d", "godunov_first_order_2d", "godunov_first_order_1d",
+                     "random_choice_1d",
                      "launch_simulation", "system_init",
                      "hfv_mh2d_create", "hfv_mh2d_advance", "hfv_mh1d_run", "hfv_probe_flux"):
         assert required in names
@@ -119,7 +120,7 @@ def test_launch_errors_before_any_device_work():
         with pytest.raises(driver.HydroError) as e:
             sim.launch(cfl=0.5, tf=1.0, **kwargs)
         assert e.value.code == code and text in e.value.text
-    sim1 = K.setup_sod(lib, 16)
+    sim1 = K.setup_inflow_1d(lib, 16)
     with pytest.raises(driver.HydroError) as e:
         sim1.launch("random_choice_1d", "riemann_solver_exact", "minmod", 0.5, 0.1)
     assert e.value.code == 6 and "random-choice" in e.value.text
@@ -145,3 +146,24 @@ def test_ensemble_rejects_first_order_configs():
     cfg.first_order = 1
     with pytest.raises(RuntimeError):
         ensemble.Ensemble(lib, [cfg])
+
+
+def test_device_config_mirrors_match_the_header():
+    """The ctypes mirrors of hfv_gpu.h's structs (hydrofv_amd.gpu,
+    hydrofv_amd.adjoint) have the C layout, as compiled by the host build in
+    tests/native."""
+    import ctypes as C
+    import os
+    from hydrofv_amd import adjoint, gpu
+    so = os.path.join(os.path.dirname(os.path.abspath(__file__)), "native", "libhfv_hostcheck.so")
+    if not os.path.exists(so):
+        pytest.fail(f"{so} is not built (make -C tests/native)")
+    L = C.CDLL(so)
+    out = (C.c_longlong * 8)()
+    L.hc_struct_layout(out)
+    assert list(out[:4]) == [C.sizeof(gpu.MH2DConfig), C.sizeof(gpu.MH1DConfig), C.sizeof(gpu.RunStatus),
+                             C.sizeof(adjoint.Adj2DConfig)]
+    assert out[4] == gpu.MH2DConfig.first_order.offset
+    assert out[5] == gpu.MH1DConfig.t_pause.offset
+    assert out[6] == gpu.MH1DConfig.random_choice.offset
+    assert out[7] == adjoint.Adj2DConfig.slope_limiter.offset

@@ -36,6 +36,7 @@ def check_run(gpulib, tmp_path, mk, integrator, solver, limiter, cfl, tf, interv
     sim = mk(gpulib)
     dim = sim.dim
     first_order = integrator.startswith("godunov")
+    random_choice = integrator.startswith("random_choice")
     res = sim.launch(integrator, solver, limiter, cfl, tf, store_dir=out, store_interval=interval,
                      store_initial=store_initial)
     files = snapshot_files(out)
@@ -48,7 +49,7 @@ def check_run(gpulib, tmp_path, mk, integrator, solver, limiter, cfl, tf, interv
         dt = float(H.read(f, "/simulation_status/dt")[0])
         start = mk(gpulib)
         grid = K.oracle_grid(start, limiter, solver)
-        r = grid.run(cfl, tf=tf, max_steps=n, first_order=first_order)
+        r = grid.run(cfl, tf=tf, max_steps=n, first_order=first_order, random_choice=random_choice)
         assert r.error == 0 and r.num_steps == n
         tol = 1e-10 if solver == "riemann_solver_exact" else 1e-12
         if solver == "riemann_solver_exact":  # pow() rounding may reach dt
@@ -68,7 +69,7 @@ def check_run(gpulib, tmp_path, mk, integrator, solver, limiter, cfl, tf, interv
         assert t >= due
         if n - 1 > prev_n:
             g2 = K.oracle_grid(mk(gpulib), limiter, solver)
-            r2 = g2.run(cfl, tf=tf, max_steps=n - 1, first_order=first_order)
+            r2 = g2.run(cfl, tf=tf, max_steps=n - 1, first_order=first_order, random_choice=random_choice)
             assert r2.t < due
         prev_n = n
     assert prev_n <= res.num_steps
@@ -115,3 +116,8 @@ def test_godunov_1d_spherical_snapshots(gpulib, tmp_path):
     check_run(gpulib, tmp_path,
               lambda L: K.setup_sod(L, 64, coord="spherical_1d", bc=("reflective", "transmissive")),
               "godunov_first_order_1d", "riemann_solver_exact", "minmod", 0.9, 0.2, 0.03)
+
+
+def test_random_choice_snapshots(gpulib, tmp_path):
+    check_run(gpulib, tmp_path, lambda L: K.setup_sod(L, 100), "random_choice_1d", "riemann_solver_exact",
+              "minmod", 0.5, 0.2, 0.05)
