@@ -22,6 +22,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, AMD public spec
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector, AMD public spec
+# wave64 VALU issue: 256 CUs x 4 SIMDs x 2.4 GHz, one FP64 wave64 instruction
+# per SIMD every 4 clocks (16 FP64 lanes per SIMD, which is the 78.6 TFLOP/s)
+VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
 ALGO_BYTES_PER_CELL = 64       # 4 FP64 conserved fields read once and written once
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "mh2d_step_kh4096_pmc.md")
 
@@ -40,6 +44,99 @@ def measured_traffic():
     except OSError:
         pass
     return None
+
+
+def measured_instructions():
+    """Per-launch instruction counters of the step kernel from the committed
+    rocprofv3 PMC summary: SQ_INSTS_VALU and its FP64 FMA/MUL/ADD parts."""
+    out = {}
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                cells = [c.strip() for c in line.split("|")]
+                if len(cells) > 3 and cells[1] in ("SQ_INSTS_VALU", "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_MUL_F64",
+                                                   "SQ_INSTS_VALU_ADD_F64"):
+                    out[cells[1]] = float(cells[2])
+    except (OSError, ValueError):
+        return None
+    return out if len(out) == 4 else None
+
+
+def compute_bound(ms_per_step):
+    """The step kernel against the FP64 VALU limits, which bound it (not
+    HBM): FP64 FLOP rate (FMA = 2) against the chip's FP64 vector peak, and
+    VALU wave-instruction issue against one FP64 wave64 instruction per SIMD
+    every 4 clocks, from the profiled instruction counts per launch."""
+    c = measured_instructions()
+    if c is None:
+        return None
+    sec = ms_per_step / 1e3
+    flops = 64 * (2 * c["SQ_INSTS_VALU_FMA_F64"] + c["SQ_INSTS_VALU_MUL_F64"] + c["SQ_INSTS_VALU_ADD_F64"])
+    tflops = flops / sec / 1e12
+    issue = c["SQ_INSTS_VALU"] / sec
+    return {"fp64_tflops": tflops, "fp64_peak_tflops": FP64_PEAK_TFLOPS, "fp64_frac": tflops / FP64_PEAK_TFLOPS,
+            "valu_wave_instr_per_s": issue, "valu_issue_peak": VALU_PEAK_WAVE_INSTR,
+            "valu_issue_frac": issue / VALU_PEAK_WAVE_INSTR, "valu_instr_per_launch": c["SQ_INSTS_VALU"],
+            "source": os.path.relpath(PMC_SUMMARY, ROOT)}
+
+
+def sod_run(lib, n=1000):
+    """Benchmark configuration 'Sod 1D, N=1000, HLLC + minmod, CFL 0.9, to
+    t=0.2' as one muscl_hancock_1d call (the whole run in one device
+    workgroup), against the reference build's serial CPU run of the same."""
+    import ctypes as C
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    from hydrofv_amd import abi, problems
+    out = {"cells": n, "tf": 0.2}
+    sim = problems.sod(lib, n)
+    sim.launch("muscl_hancock_1d", "riemann_solver_hllc", "minmod", 0.9, 0.2)  # warm-up
+    sim.close()
+    sim = problems.sod(lib, n)
+    t0 = time.perf_counter()
+    res = sim.launch("muscl_hancock_1d", "riemann_solver_hllc", "minmod", 0.9, 0.2)
+    out["device_ms"] = (time.perf_counter() - t0) * 1e3
+    out["steps"] = res.num_steps
+    sim.close()
+    if os.path.exists(O.REF_SO):
+        ref = abi.bind(C.CDLL(O.REF_SO))
+        sim = problems.sod(ref, n)
+        t0 = time.perf_counter()
+        rres = sim.launch("muscl_hancock_1d", "riemann_solver_hllc", "minmod", 0.9, 0.2)
+        out["cpu_reference_serial_ms"] = (time.perf_counter() - t0) * 1e3
+        out["cpu_steps"] = rres.num_steps
+        sim.close()
+    out["note"] = "wall time of the library call, host-device copies included"
+    return out
+
+
+def sedov_run(lib, n=2048, steps=20):
+    """Benchmark configuration 'Sedov 2D 2048^2, exact Riemann solver (tol
+    1e-8) + Van Leer, CFL 0.5': cell-updates/s over `steps` steps after 10
+    warm-up steps, state resident on the device."""
+    import torch
+    from hydrofv_amd import gpu, problems
+    sim = problems.sedov(lib, n)
+    cfg = problems.device_config(sim, limiter=2, solver=1, cfl=0.5, tf=0.05, tol=1e-8)
+    dev = gpu.MH2D(lib, cfg)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    dev.upload(sim.primitive(), sim.conserved(), stream=sh)
+    sim.close()
+    dev.start(stream=sh)
+    dev.advance(9, use_graph=False, stream=sh)
+    st0 = dev.status(stream=sh)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    dev.advance(steps, use_graph=False, stream=sh)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    st = dev.status(stream=sh)
+    dev.close()
+    assert st.err_kind == 0 and st.num_steps == st0.num_steps + steps, (st.err_kind, st.num_steps)
+    return {"cells": f"{n}x{n}", "solver": "exact (tol 1e-8)", "limiter": "van_leer", "steps": steps,
+            "ms_per_step": ms / steps, "cell_updates_per_s": n * n * steps / (ms / 1e3), "t_after": st.t}
 
 
 def cpu_baseline(n: int, t_sample: float, steps_sample: int):
@@ -94,6 +191,8 @@ def main():
                     help="gradient solves of the 1024^2, 500-step adjoint workload (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=96,
                     help="steps of the 256-instance batched ensemble and of its sequential baseline (0: skip)")
+    ap.add_argument("--no-extra-configs", action="store_true",
+                    help="skip the Sod 1D and Sedov 2D configuration runs")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB once on the device: a "
                          "kernel of known traffic that scales the profiler's FETCH_SIZE/WRITE_SIZE counters")
@@ -194,6 +293,15 @@ def main():
         except Exception as e:  # the main figure stands without it
             log(f"adjoint benchmark failed: {e!r}")
 
+    extra = {}
+    if not args.no_extra_configs:
+        for name, fn in (("sod_1d", sod_run), ("sedov_2d", sedov_run)):
+            try:
+                extra[name] = fn(lib)
+                log(f"{name}: {extra[name]}")
+            except Exception as e:  # the main figure stands without them
+                log(f"{name} failed: {e!r}")
+
     traffic = measured_traffic()
     line = {
         "metric": "cell-updates/s FP64 MUSCL-HLLC 4096²; % HBM roofline; batched & adjoint ratio",
@@ -213,9 +321,11 @@ def main():
                    "cfl": 0.8, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "compute_bound": compute_bound(ms_per_step),
         "cpu_baseline": cpu,
         "ensemble": ens,
         "adjoint": adj,
+        **extra,
     }
     print(json.dumps(line), flush=True)
 

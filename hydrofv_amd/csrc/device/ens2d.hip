@@ -52,43 +52,46 @@ __global__ void __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(SOLV
     strip_finish(r, out, lane);
 }
 
-// per-instance controller reset and first-dt reduction from W (all instances in one launch)
-__global__ void k_ens_init(const EnsInstance* __restrict__ insts, int count)
+// per-instance controller reset (three slots each)
+__global__ void k_ens_reset(const EnsInstance* __restrict__ insts, int count)
 {
-    const int i = blockIdx.x;
-    if (i >= count) return;
-    const EnsInstance& E = insts[i];
-    if (threadIdx.x < 3) {
-        Ctrl& c = E.ctrl[threadIdx.x];
-        c.t = 0.0;
-        c.dt = 0.0;
-        c.t_prev = 0.0;
-        c.num_steps = 0;
-        c.key[0] = 0ull;
-        c.key[1] = 0ull;
-        c.cur = 0;
-        c.err_kind = ERR_NONE;
-        c.err_cell = -1;
-    }
-    __syncthreads();
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 3 * count) return;
+    Ctrl& c = insts[k / 3].ctrl[k % 3];
+    c.t = 0.0;
+    c.dt = 0.0;
+    c.t_prev = 0.0;
+    c.num_steps = 0;
+    c.key[0] = 0ull;
+    c.key[1] = 0ull;
+    c.cur = 0;
+    c.err_kind = ERR_NONE;
+    c.err_cell = -1;
+}
+
+// first-dt CFL maxima from every instance's W (utils.c:55-95): workgroup
+// (instance, b) strides over rows b, b + ENS_SMAX_ROW_BLOCKS, ... of that
+// instance and adds one atomic per key
+constexpr int ENS_SMAX_THREADS = 256;
+constexpr int ENS_SMAX_ROW_BLOCKS = 16;
+
+__global__ void __launch_bounds__(ENS_SMAX_THREADS) k_ens_smax(const EnsInstance* __restrict__ insts, int count)
+{
+    const EnsInstance& E = insts[blockIdx.x];
     const Grid2D& G = E.G;
     double sx = -INFINITY, sy = -INFINITY;
-    for (long long k = threadIdx.x; k < (long long)G.nx * G.ny; k += blockDim.x) {
-        const int j = (int)(k / G.nx) + G.g, ii = (int)(k % G.nx) + G.g;
-        const long long q = (long long)j * G.pitch + ii;
-        const double a = sound_speed(G.gamma, E.W.f[0][q], E.W.f[3][q]);
-        const double vx = a + fabs(E.W.f[1][q]);
-        const double vy = a + fabs(E.W.f[2][q]);
-        if (vx > sx) sx = vx;
-        if (vy > sy) sy = vy;
+    for (int j = blockIdx.y; j < G.ny; j += gridDim.y) {
+        const long long row = (long long)(j + G.g) * G.pitch + G.g;
+        for (int i = threadIdx.x; i < G.nx; i += ENS_SMAX_THREADS) {
+            const long long q = row + i;
+            const double a = sound_speed(G.gamma, E.W.f[0][q], E.W.f[3][q]);
+            const double vx = a + fabs(E.W.f[1][q]);
+            const double vy = a + fabs(E.W.f[2][q]);
+            if (vx > sx) sx = vx;
+            if (vy > sy) sy = vy;
+        }
     }
-    sx = wave_max(sx);
-    sy = wave_max(sy);
-    if ((threadIdx.x & 63) == 0) {
-        const unsigned long long kx = smax_key(sx), ky = smax_key(sy);
-        if (kx) atomicMax(&E.ctrl[0].key[0], kx);
-        if (ky) atomicMax(&E.ctrl[0].key[1], ky);
-    }
+    block_smax_to<ENS_SMAX_THREADS>(E.ctrl, sx, sy);
 }
 
 }  // namespace hfv
@@ -342,7 +345,9 @@ int hfv_ens_start(hfv_ens* e, void* stream)
 {
     hipStream_t s = stream ? (hipStream_t)stream : e->own;
     e->launches = 0;
-    hipLaunchKernelGGL(k_ens_init, dim3(e->count), dim3(256), 0, s, e->dinst, e->count);
+    hipLaunchKernelGGL(k_ens_reset, dim3((3 * e->count + 255) / 256), dim3(256), 0, s, e->dinst, e->count);
+    hipLaunchKernelGGL(k_ens_smax, dim3(e->count, ENS_SMAX_ROW_BLOCKS), dim3(ENS_SMAX_THREADS), 0, s, e->dinst,
+                       e->count);
     ens_launch(e, true, s);
     return hipGetLastError() == hipSuccess ? HFV_OK : HFV_EDEVICE;
 }

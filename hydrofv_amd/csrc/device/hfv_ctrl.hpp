@@ -123,6 +123,33 @@ __device__ __forceinline__ void publish(const Ctrl* in, Ctrl* out, Ctrl* clear, 
 }
 
 // wave64 reductions ---------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v);
+
+// Workgroup maxima of (sx, sy), folded into c's CFL keys by one atomic per key
+// (THREADS a multiple of 64; every thread of the workgroup calls this).
+template <int THREADS>
+__device__ __forceinline__ void block_smax_to(Ctrl* c, double sx, double sy)
+{
+    sx = wave_max(sx);
+    sy = wave_max(sy);
+    __shared__ double red[2][THREADS / 64];
+    const int w = threadIdx.x / 64;
+    if (threadIdx.x % 64 == 0) {
+        red[0][w] = sx;
+        red[1][w] = sy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < THREADS / 64; q++) {
+            if (red[0][q] > sx) sx = red[0][q];
+            if (red[1][q] > sy) sy = red[1][q];
+        }
+        const unsigned long long kx = smax_key(sx), ky = smax_key(sy);
+        if (kx) atomicMax(&c->key[0], kx);
+        if (ky) atomicMax(&c->key[1], ky);
+    }
+}
+
 __device__ __forceinline__ double wave_max(double v)
 {
 #pragma unroll

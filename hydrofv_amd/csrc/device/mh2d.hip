@@ -51,24 +51,7 @@ __global__ void __launch_bounds__(SMAX_THREADS) k_smax_from_w(Grid2D G, Fields4 
             if (vy > sy) sy = vy;
         }
     }
-    sx = wave_max(sx);
-    sy = wave_max(sy);
-    __shared__ double red[2][SMAX_THREADS / LANES];
-    const int w = threadIdx.x / LANES;
-    if (threadIdx.x % LANES == 0) {
-        red[0][w] = sx;
-        red[1][w] = sy;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int q = 1; q < SMAX_THREADS / LANES; q++) {
-            if (red[0][q] > sx) sx = red[0][q];
-            if (red[1][q] > sy) sy = red[1][q];
-        }
-        const unsigned long long kx = smax_key(sx), ky = smax_key(sy);
-        if (kx) atomicMax(&ctrl[slot].key[0], kx);
-        if (ky) atomicMax(&ctrl[slot].key[1], ky);
-    }
+    block_smax_to<SMAX_THREADS>(ctrl + slot, sx, sy);
 }
 
 // Primitive state of every cell after a step: c2p everywhere, then the ghost
