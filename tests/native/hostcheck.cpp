@@ -26,6 +26,25 @@ void limits(long long n, const double* a, const double* b, double* d, double* da
 
 extern "C" {
 
+// the exact solver's sampler with (shortcut = 1) or without its equal-state
+// shortcut: w[n][4] = (rho, u, v, p) left and right states, speed[n]
+void hc_exact_sample(int shortcut, double gamma, double tol, long long n, const double* wl, const double* wr,
+                     const double* speed, double* out, double* us, int* rc)
+{
+    for (long long i = 0; i < n; i++) {
+        const double* l = wl + 4 * i;
+        const double* r = wr + 4 * i;
+        const Sample S = shortcut ? exact_sample<true>(gamma, tol, l[0], l[1], l[2], l[3], r[0], r[1], r[2], r[3], speed[i])
+                                  : exact_sample<false>(gamma, tol, l[0], l[1], l[2], l[3], r[0], r[1], r[2], r[3], speed[i]);
+        out[4 * i + 0] = S.w.r;
+        out[4 * i + 1] = S.w.u;
+        out[4 * i + 2] = S.w.v;
+        out[4 * i + 3] = S.w.p;
+        us[i] = S.us;
+        rc[i] = S.rc;
+    }
+}
+
 // sizes and selected field offsets of the structs of hfv_gpu.h, for the
 // ctypes mirrors: out[0..3] sizeof(mh2d, mh1d, run_status, adj2d config),
 // out[4..] offsets of the fields added last to each config

@@ -72,12 +72,16 @@ def _trrs_guess(wl, wr):
 
 
 def pow_dependent(wl, wr):
-    """Faces whose HLLC flux depends on the last power of the two-rarefaction
-    guess (p* not clearly below both pressures): there the kernels' pow_pos
+    """Faces whose HLLC flux depends on a power: the last power of the
+    two-rarefaction guess (p* not clearly below both pressures) or the
+    vacuum samplers (riemann_solver.c:239-733).  There the kernels' pow_pos
     may differ from glibc's pow in the last place."""
     trrs, guess, plo = trrs_guess(wl, wr)
-    with np.errstate(invalid="ignore"):
-        return trrs & ~(guess < plo * (1 - 1e-9))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        al = np.where(wl[:, 0] > 0, np.sqrt(GAMMA * wl[:, 3] / wl[:, 0]), 0.0)
+        ar = np.where(wr[:, 0] > 0, np.sqrt(GAMMA * wr[:, 3] / wr[:, 0]), 0.0)
+        vac = (wl[:, 0] <= 0) | (wr[:, 0] <= 0) | ((al + ar) * 2.0 / (GAMMA + 1.0) <= (wr[:, 1] - wl[:, 1]))
+        return (trrs & ~(guess < plo * (1 - 1e-9))) | vac
 
 
 def assert_flux_parity(f_dev, f_ref, dep):
@@ -193,6 +197,10 @@ def test_pow_pos_accuracy(hc):
 
 @pytest.mark.parametrize("nf", [3, 4])
 def test_host_exact_equals_oracle(hc, nf):
+    """The exact solver's powers come from pow_pos, which differs from
+    glibc's pow in the last place on about 1 argument in 1000: the fluxes
+    are bitwise on all but a few faces and within 1e-14 of the largest
+    component there; the same faces fail."""
     wl, wr = K.riemann_states(3000, 13)
     if nf == 3:
         wl, wr = wl[:, [0, 1, 3]].copy(), wr[:, [0, 1, 3]].copy()
@@ -203,7 +211,15 @@ def test_host_exact_equals_oracle(hc, nf):
     fo, eo = O.flux_batch(2 if nf == 4 else 1, "riemann_solver_exact", GAMMA, 1e-8, wl, wr)
     ok = eo == 0
     assert ok.sum() > n // 2
-    assert np.array_equal(f[ok], fo[ok], equal_nan=True)
+    assert np.array_equal(err != 0, eo != 0)
+    a, b = f[ok], fo[ok]
+    same = np.all((a == b) | (np.isnan(a) & np.isnan(b)), axis=1)
+    assert same.mean() > 0.99, same.mean()
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    scale = np.nanmax(np.abs(b), axis=1, keepdims=True)
+    with np.errstate(invalid="ignore"):
+        rel = np.abs(a - b) / np.maximum(scale, 1e-300)
+    assert np.nanmax(rel) <= 1e-14, np.nanmax(rel)
 
 
 # ---------------------------------------------------------------------------
@@ -342,3 +358,40 @@ def test_torch_step_matches_oracle():
     err = np.abs(U.numpy() - ref).max(axis=(1, 2)) / np.abs(ref).max(axis=(1, 2))
     assert err.max() < 1e-14, err
     sim.close()
+
+
+def test_exact_solver_equal_state_shortcut_is_bitwise(hc):
+    """The exact solver's shortcut for equal left and right states returns
+    the bits of the full solve (Newton, u*, sampler), sign of zero included,
+    for states across the double range, pressures below and above tol, and
+    sample speeds on every side of the waves."""
+    hc.hc_exact_sample.argtypes = [C.c_int, C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp, _dp, _dp, _ip]
+    r = np.random.default_rng(11)
+    n = 30_000
+    rho = np.exp(r.uniform(np.log(1e-300), np.log(1e300), n))
+    p = np.exp(r.uniform(np.log(1e-300), np.log(1e300), n))
+    p[: n // 4] = np.exp(r.uniform(np.log(1e-12), np.log(1e3), n // 4))
+    u = r.choice([-1.0, 1.0], n) * np.exp(r.uniform(np.log(1e-300), np.log(1e300), n))
+    u[r.uniform(size=n) < 0.05] = 0.0
+    u[r.uniform(size=n) < 0.05] = -0.0
+    v = r.normal(size=n)
+    wl = np.stack([rho, u, v, p], axis=1)
+    wr = wl.copy()
+    # +0 / -0 velocities on the two sides compare equal but differ in sign
+    flip = (u == 0) & (r.uniform(size=n) < 0.5)
+    wr[flip, 1] = -wr[flip, 1]
+    for gamma in (1.4, 5.0 / 3.0):
+        a = np.sqrt(gamma * p / rho)
+        speed = np.select([r.uniform(size=n) < 0.2, r.uniform(size=n) < 0.4, r.uniform(size=n) < 0.6],
+                          [np.zeros(n), u - a, u + a], u + a * r.uniform(-3, 3, n))
+        for tol in (1e-8,):
+            outs = []
+            for shortcut in (1, 0):
+                out = np.empty((n, 4))
+                us = np.empty(n)
+                rc = np.empty(n, dtype=np.int32)
+                hc.hc_exact_sample(shortcut, gamma, tol, n, wl, wr, speed, out, us, rc)
+                outs.append((out.view(np.int64), us.view(np.int64), rc))
+            assert np.array_equal(outs[0][0], outs[1][0]), gamma
+            assert np.array_equal(outs[0][1], outs[1][1]), gamma
+            assert np.array_equal(outs[0][2], outs[1][2]), gamma
