@@ -88,6 +88,71 @@ void prim_from_u(const Grid2D& G, const Fields4& U, const Fields4& W, hipStream_
     hipLaunchKernelGGL(k_prim_from_u, grid, block, 0, s, G, U, W, fill);
 }
 
+// c2p of every cell (system.c:763-802) from the conserved buffer that holds
+// the current state, as named by the controller slot the next launch reads
+__global__ void k_prim_from_slot(Grid2D G, Fields4 Ua, Fields4 Ub, Fields4 W, const Ctrl* ctrl, int slot)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (c >= G.tx) return;
+    const Fields4 U = ctrl[slot].cur ? Ub : Ua;
+    const long long k = (long long)q * G.pitch + c;
+    double w[4];
+    c2p_2d(G.gamma, G.volume, U.f[0][k], U.f[1][k], U.f[2][k], U.f[3][k], w);
+    W.f[0][k] = w[0];
+    W.f[1][k] = w[1];
+    W.f[2][k] = w[2];
+    W.f[3][k] = w[3];
+}
+
+// Built-in boundary fills of the primitive buffer, from its own interior
+// cells (boundary.c:513-777): the x sides on the interior rows, the y sides
+// on the interior columns.  Each side reads interior cells only and writes
+// its own ghost cells, so the sides in `mask` (bit s = side s, order x_min,
+// x_max, y_min, y_max) run in one launch.  A periodic pair is filled from
+// its max side, both ghost strips, as in the reference; the min side's
+// periodic case does nothing.  Needs num_cells >= ghost cells per direction.
+__global__ void k_fill_sides(Grid2D G, Fields4 W, int mask)
+{
+    const int side = blockIdx.y;
+    if (!((mask >> side) & 1)) return;
+    const int bc = G.bc[side];
+    const bool x = side < 2;
+    const int span = x ? G.ny : G.nx;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= G.g * span) return;
+    const int layer = e / span, along = G.g + e % span;
+    const int n = x ? G.nx : G.ny;
+    const int flip = x ? 1 : 2;  // the normal velocity
+    // (ghost index, source index) pairs along the normal; at most two for a
+    // periodic pair
+    int dst[2], src[2], cnt = 0;
+    if (bc == BC_REFLECTIVE || bc == BC_TRANSMISSIVE) {
+        if (side == 0 || side == 2) {
+            dst[0] = G.g - 1 - layer;
+            src[0] = G.g + layer;
+        } else {
+            dst[0] = G.g + n + layer;
+            src[0] = G.g + n - 1 - layer;
+        }
+        cnt = 1;
+    } else if (bc == BC_PERIODIC && (side == 1 || side == 3)) {
+        dst[0] = G.g + n + layer;
+        src[0] = G.g + layer;
+        dst[1] = G.g - 1 - layer;
+        src[1] = G.g + n - 1 - layer;
+        cnt = 2;
+    }
+    for (int m = 0; m < cnt; m++) {
+        const long long kd = x ? (long long)along * G.pitch + dst[m] : (long long)dst[m] * G.pitch + along;
+        const long long ks = x ? (long long)along * G.pitch + src[m] : (long long)src[m] * G.pitch + along;
+        for (int f = 0; f < 4; f++) {
+            const double v = W.f[f][ks];
+            W.f[f][kd] = (bc == BC_REFLECTIVE && f == flip) ? -v : v;
+        }
+    }
+}
+
 __global__ void k_clear_keys(Ctrl* ctrl, int slot)
 {
     if (threadIdx.x == 0) {
@@ -136,11 +201,49 @@ struct hfv_mh2d {
     hipStream_t graph_stream;
     hipStream_t own;       // used when the caller passes no stream (graphs cannot be captured on the null stream)
     int w_pending;         // the next step reads the primitive state set by hfv_mh2d_set_primitive
+    hfv_device_bc_2d_func dev_fn[4];  // device hooks of the custom sides
+    void* dev_user[4];
+    long long w_filled;    // launch count at which the primitive buffer was last filled (device hooks)
+    double* stage;         // one contiguous field for host copies (allocated at first use)
 };
 
 namespace {
 
 hipStream_t pick(hfv_mh2d* h, void* stream) { return stream ? (hipStream_t)stream : h->own; }
+
+// Host copies of one field (reference layout, tx x ty, x fastest) from and
+// to a pitched device field.  A pitched copy to pageable host memory goes
+// row by row, so unless the pitch equals the width the field passes through
+// a contiguous device buffer: a device-side 2D copy plus one linear copy.
+bool staged(hfv_mh2d* h)
+{
+    if (h->G.pitch == h->G.tx) return false;
+    if (!h->stage && hipMalloc(&h->stage, (size_t)h->G.tx * h->G.ty * sizeof(double)) != hipSuccess) {
+        h->stage = nullptr;
+        (void)hipGetLastError();
+    }
+    return h->stage != nullptr;
+}
+
+hipError_t field_to_host(hfv_mh2d* h, double* host, const double* dev, hipStream_t s)
+{
+    const Grid2D& G = h->G;
+    const size_t hp = (size_t)G.tx * sizeof(double), dp = (size_t)G.pitch * sizeof(double);
+    if (G.pitch == G.tx) return hipMemcpyAsync(host, dev, hp * G.ty, hipMemcpyDeviceToHost, s);
+    if (!staged(h)) return hipMemcpy2DAsync(host, hp, dev, dp, hp, G.ty, hipMemcpyDeviceToHost, s);
+    hipError_t e = hipMemcpy2DAsync(h->stage, hp, dev, dp, hp, G.ty, hipMemcpyDeviceToDevice, s);
+    return e != hipSuccess ? e : hipMemcpyAsync(host, h->stage, hp * G.ty, hipMemcpyDeviceToHost, s);
+}
+
+hipError_t field_to_device(hfv_mh2d* h, double* dev, const double* host, hipStream_t s)
+{
+    const Grid2D& G = h->G;
+    const size_t hp = (size_t)G.tx * sizeof(double), dp = (size_t)G.pitch * sizeof(double);
+    if (G.pitch == G.tx) return hipMemcpyAsync(dev, host, hp * G.ty, hipMemcpyHostToDevice, s);
+    if (!staged(h)) return hipMemcpy2DAsync(dev, dp, host, hp, hp, G.ty, hipMemcpyHostToDevice, s);
+    hipError_t e = hipMemcpyAsync(h->stage, host, hp * G.ty, hipMemcpyHostToDevice, s);
+    return e != hipSuccess ? e : hipMemcpy2DAsync(dev, dp, h->stage, hp, hp, G.ty, hipMemcpyDeviceToDevice, s);
+}
 
 template <int LIM, int SOLVER, bool BRANCHY>
 void launch_step_t(hfv_mh2d* h, bool wmode, hipStream_t s)
@@ -179,6 +282,59 @@ void launch_step(hfv_mh2d* h, bool wmode, hipStream_t s)
         else if (lim == VAN_LEER) launch_step_t<VAN_LEER, EXACT, false>(h, wmode, s);
         else launch_step_t<MC, EXACT, false>(h, wmode, s);
     }
+}
+
+int device_bc_mode(const hfv_mh2d* h)
+{
+    for (int s = 0; s < 4; s++)
+        if (h->dev_fn[s]) return 1;
+    return 0;
+}
+
+int device_bc_complete(const hfv_mh2d* h)
+{
+    for (int s = 0; s < 4; s++)
+        if (h->G.bc[s] == BC_CUSTOM && !h->dev_fn[s]) return 0;
+    return 1;
+}
+
+void fill_sides(hfv_mh2d* h, int mask, hipStream_t s)
+{
+    if (!mask) return;
+    const Grid2D& G = h->G;
+    const int span = std::max(G.nx, G.ny);
+    const dim3 grid((unsigned)((G.g * span + 255) / 256), 4), block(256);
+    hipLaunchKernelGGL(k_fill_sides, grid, block, 0, s, G, h->W, mask);
+}
+
+// Device-hook mode: the primitive buffer for the next launch, i.e. the
+// reference's c2p then set_boundary_condition (boundary.c:513-777, sides in
+// its order) at the end of a step, then the CFL maxima of that state
+// (utils.c:55-95) into the slot the next launch reads.  Once per launch
+// count, so a snapshot's download and the next step share one fill.
+void fill_primitive(hfv_mh2d* h, hipStream_t s)
+{
+    if (h->w_filled == h->launches) return;
+    const Grid2D& G = h->G;
+    const int slot = (int)(h->launches % 3);
+    hipLaunchKernelGGL(k_prim_from_slot, dim3((G.tx + 63) / 64, G.ty), dim3(64), 0, s, G, h->U[0], h->U[1], h->W,
+                       h->ctrl, slot);
+    int mask = 0;
+    for (int side = 0; side < 4; side++) {
+        if (G.bc[side] != BC_CUSTOM) {
+            mask |= 1 << side;
+            continue;
+        }
+        fill_sides(h, mask, s);
+        mask = 0;
+        h->dev_fn[side](h->W.f[0], h->W.f[1], h->W.f[2], h->W.f[3], G.g, G.nx, G.ny, G.pitch, (void*)s,
+                        h->dev_user[side]);
+    }
+    fill_sides(h, mask, s);
+    hipLaunchKernelGGL(k_clear_keys, dim3(1), dim3(64), 0, s, h->ctrl, slot);
+    const dim3 grid((unsigned)std::min(G.ny, 2048)), block(SMAX_THREADS);
+    hipLaunchKernelGGL(k_smax_from_w, grid, block, 0, s, G, h->W, h->ctrl, slot);
+    h->w_filled = h->launches;
 }
 
 }  // namespace
@@ -245,6 +401,7 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
     h->launches = 0;
     h->graph = nullptr;
     h->graph_len = 0;
+    h->w_filled = -1;
     if (hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess) {
         (void)hipFree(h->slab);
         delete h;
@@ -260,6 +417,7 @@ void hfv_mh2d_destroy(hfv_mh2d* h)
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
     (void)hipStreamSynchronize(h->own);
     (void)hipStreamDestroy(h->own);
+    if (h->stage) (void)hipFree(h->stage);
     (void)hipFree(h->slab);
     delete h;
 }
@@ -268,15 +426,14 @@ int hfv_mh2d_upload(hfv_mh2d* h, const double* const W[4], const double* const U
 {
     hipStream_t s = pick(h, stream);
     const Grid2D& G = h->G;
-    const size_t hp = (size_t)G.tx * sizeof(double), dp = (size_t)G.pitch * sizeof(double);
+    const size_t dp = (size_t)G.pitch * sizeof(double), wb = (size_t)G.tx * sizeof(double);
     for (int f = 0; f < 4; f++) {
-        if (hipMemcpy2DAsync(h->W.f[f], dp, W[f], hp, hp, G.ty, hipMemcpyHostToDevice, s) != hipSuccess)
-            return HFV_EDEVICE;
+        if (field_to_device(h, h->W.f[f], W[f], s) != hipSuccess) return HFV_EDEVICE;
         // both buffers start from the same conserved state: cells a step does
         // not write (deeper ghost layers, corners) stay valid in either one
-        if (hipMemcpy2DAsync(h->U[0].f[f], dp, U[f], hp, hp, G.ty, hipMemcpyHostToDevice, s) != hipSuccess)
-            return HFV_EDEVICE;
-        if (hipMemcpy2DAsync(h->U[1].f[f], dp, U[f], hp, hp, G.ty, hipMemcpyHostToDevice, s) != hipSuccess)
+        if (field_to_device(h, h->U[0].f[f], U[f], s) != hipSuccess) return HFV_EDEVICE;
+        if (hipMemcpy2DAsync(h->U[1].f[f], dp, h->U[0].f[f], dp, wb, G.ty, hipMemcpyDeviceToDevice, s) !=
+            hipSuccess)
             return HFV_EDEVICE;
     }
     return HFV_OK;
@@ -286,6 +443,7 @@ int hfv_mh2d_start(hfv_mh2d* h, void* stream)
 {
     hipStream_t s = pick(h, stream);
     h->launches = 0;
+    h->w_filled = -1;
     hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, h->ctrl, 0.0, 0LL);
     const dim3 grid((unsigned)std::min(h->G.ny, 2048)), block(SMAX_THREADS);
     hipLaunchKernelGGL(k_smax_from_w, grid, block, 0, s, h->G, h->W, h->ctrl, 0);
@@ -296,6 +454,17 @@ int hfv_mh2d_start(hfv_mh2d* h, void* stream)
 int hfv_mh2d_advance(hfv_mh2d* h, long long launches, int use_graph, void* stream)
 {
     hipStream_t s = pick(h, stream);
+    if (device_bc_mode(h)) {
+        // every step reads the primitive buffer that the device fill, hooks
+        // included, has just refreshed
+        if (!device_bc_complete(h)) return HFV_EINVAL;
+        for (long long i = 0; i < launches; i++) {
+            if (h->w_pending) h->w_pending = 0;
+            else fill_primitive(h, s);
+            launch_step(h, true, s);
+        }
+        return hipGetLastError() == hipSuccess ? HFV_OK : HFV_EDEVICE;
+    }
     if (h->w_pending && launches > 0) {
         launch_step(h, true, s);
         h->w_pending = 0;
@@ -380,13 +549,15 @@ int hfv_mh2d_download(hfv_mh2d* h, double* const W[4], double* const U[4], void*
     const Grid2D& G = h->G;
     const Fields4 cur = h->U[st.cur];
     // only when asked for: W may hold a state set by hfv_mh2d_set_primitive
-    if (W) prim_from_u(G, cur, h->W, s);
-    const size_t hp = (size_t)G.tx * sizeof(double), dp = (size_t)G.pitch * sizeof(double);
+    if (W && device_bc_mode(h)) {
+        if (!device_bc_complete(h)) return HFV_EINVAL;
+        if (!h->w_pending) fill_primitive(h, s);
+    } else if (W) {
+        prim_from_u(G, cur, h->W, s);
+    }
     for (int f = 0; f < 4; f++) {
-        if (W && hipMemcpy2DAsync(W[f], hp, h->W.f[f], dp, hp, G.ty, hipMemcpyDeviceToHost, s) != hipSuccess)
-            return HFV_EDEVICE;
-        if (U && hipMemcpy2DAsync(U[f], hp, cur.f[f], dp, hp, G.ty, hipMemcpyDeviceToHost, s) != hipSuccess)
-            return HFV_EDEVICE;
+        if (W && field_to_host(h, W[f], h->W.f[f], s) != hipSuccess) return HFV_EDEVICE;
+        if (U && field_to_host(h, U[f], cur.f[f], s) != hipSuccess) return HFV_EDEVICE;
     }
     return hipStreamSynchronize(s) == hipSuccess ? HFV_OK : HFV_EDEVICE;
 }
@@ -399,10 +570,8 @@ int hfv_mh2d_primitive(hfv_mh2d* h, double* const W[4], void* stream)
     if (rc) return rc;
     const Grid2D& G = h->G;
     prim_from_u(G, h->U[st.cur], h->W, s, 0);
-    const size_t hp = (size_t)G.tx * sizeof(double), dp = (size_t)G.pitch * sizeof(double);
     for (int f = 0; f < 4; f++)
-        if (hipMemcpy2DAsync(W[f], hp, h->W.f[f], dp, hp, G.ty, hipMemcpyDeviceToHost, s) != hipSuccess)
-            return HFV_EDEVICE;
+        if (field_to_host(h, W[f], h->W.f[f], s) != hipSuccess) return HFV_EDEVICE;
     return hipStreamSynchronize(s) == hipSuccess ? HFV_OK : HFV_EDEVICE;
 }
 
@@ -410,10 +579,8 @@ int hfv_mh2d_set_primitive(hfv_mh2d* h, const double* const W[4], void* stream)
 {
     hipStream_t s = pick(h, stream);
     const Grid2D& G = h->G;
-    const size_t hp = (size_t)G.tx * sizeof(double), dp = (size_t)G.pitch * sizeof(double);
     for (int f = 0; f < 4; f++)
-        if (hipMemcpy2DAsync(h->W.f[f], dp, W[f], hp, hp, G.ty, hipMemcpyHostToDevice, s) != hipSuccess)
-            return HFV_EDEVICE;
+        if (field_to_device(h, h->W.f[f], W[f], s) != hipSuccess) return HFV_EDEVICE;
     // the next step's CFL maxima come from this state (utils.c:55-95)
     const int slot = (int)(h->launches % 3);
     hipLaunchKernelGGL(k_clear_keys, dim3(1), dim3(64), 0, s, h->ctrl, slot);
@@ -422,6 +589,15 @@ int hfv_mh2d_set_primitive(hfv_mh2d* h, const double* const W[4], void* stream)
     h->w_pending = 1;
     if (hipGetLastError() != hipSuccess) return HFV_EDEVICE;
     return hipStreamSynchronize(s) == hipSuccess ? HFV_OK : HFV_EDEVICE;
+}
+
+int hfv_mh2d_set_device_bc(hfv_mh2d* h, int side, hfv_device_bc_2d_func fn, void* user)
+{
+    if (side < 0 || side > 3 || h->G.bc[side] != BC_CUSTOM) return HFV_EINVAL;
+    h->dev_fn[side] = fn;
+    h->dev_user[side] = fn ? user : nullptr;
+    h->w_filled = -1;
+    return HFV_OK;
 }
 
 const char* hfv_device_error_string(void)

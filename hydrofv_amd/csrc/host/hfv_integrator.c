@@ -9,6 +9,7 @@
  * conserved arrays back.  SimulationStatus and the error codes/messages follow
  * the reference.
  */
+#include <pthread.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -87,6 +88,71 @@ static int has_custom_bc(const BoundaryConditionParam *bc, int two_d)
            bc->boundary_condition_flag_x_max_ == BOUNDARY_CONDITION_CUSTOM ||
            (two_d && (bc->boundary_condition_flag_y_min_ == BOUNDARY_CONDITION_CUSTOM ||
                       bc->boundary_condition_flag_y_max_ == BOUNDARY_CONDITION_CUSTOM));
+}
+
+/* Device forms of custom 2D hooks, keyed by the host hook they stand for. */
+#define DEVICE_BC_MAX 64
+typedef struct {
+    set_boundary_condition_cartesian_2d_func host;
+    hfv_device_bc_2d_func device;
+    void *user;
+} DeviceBc;
+static DeviceBc device_bc[DEVICE_BC_MAX];
+static int device_bc_count;
+static pthread_mutex_t device_bc_lock = PTHREAD_MUTEX_INITIALIZER;
+
+int hfv_register_device_bc_2d(set_boundary_condition_cartesian_2d_func host_hook, hfv_device_bc_2d_func device_hook,
+                              void *user)
+{
+    if (!host_hook) return -1;
+    int rc = 0;
+    pthread_mutex_lock(&device_bc_lock);
+    int k = 0;
+    while (k < device_bc_count && device_bc[k].host != host_hook) k++;
+    if (!device_hook) {
+        if (k < device_bc_count) device_bc[k] = device_bc[--device_bc_count];
+    } else if (k < device_bc_count || device_bc_count < DEVICE_BC_MAX) {
+        if (k == device_bc_count) device_bc_count++;
+        device_bc[k].host = host_hook;
+        device_bc[k].device = device_hook;
+        device_bc[k].user = user;
+    } else {
+        rc = -1;
+    }
+    pthread_mutex_unlock(&device_bc_lock);
+    return rc;
+}
+
+static int find_device_bc(set_boundary_condition_cartesian_2d_func host_hook, DeviceBc *out)
+{
+    int found = 0;
+    pthread_mutex_lock(&device_bc_lock);
+    for (int k = 0; k < device_bc_count && !found; k++) {
+        if (device_bc[k].host == host_hook) {
+            *out = device_bc[k];
+            found = 1;
+        }
+    }
+    pthread_mutex_unlock(&device_bc_lock);
+    return found;
+}
+
+/* Gives every custom side of a 2D run its registered device hook; 0 (and
+ * none set) when some custom side has none, so the run keeps the host path. */
+static int attach_device_bc(const BoundaryConditionParam *bc, hfv_mh2d *h)
+{
+    const int flags[4] = {bc->boundary_condition_flag_x_min_, bc->boundary_condition_flag_x_max_,
+                          bc->boundary_condition_flag_y_min_, bc->boundary_condition_flag_y_max_};
+    const set_boundary_condition_cartesian_2d_func hooks[4] = {
+        bc->set_boundary_condition_cartesian_2d_x_min, bc->set_boundary_condition_cartesian_2d_x_max,
+        bc->set_boundary_condition_cartesian_2d_y_min, bc->set_boundary_condition_cartesian_2d_y_max};
+    DeviceBc found[4];
+    for (int s = 0; s < 4; s++)
+        if (flags[s] == BOUNDARY_CONDITION_CUSTOM && !find_device_bc(hooks[s], &found[s])) return 0;
+    for (int s = 0; s < 4; s++)
+        if (flags[s] == BOUNDARY_CONDITION_CUSTOM && hfv_mh2d_set_device_bc(h, s, found[s].device, found[s].user) != HFV_OK)
+            return 0;
+    return 1;
 }
 
 /* Snapshot bookkeeping of the reference loop (integrator_muscl_hancock.c:
@@ -179,15 +245,17 @@ static ErrorStatus run_2d(BoundaryConditionParam *bc, System *system, Integrator
     memset(&st, 0, sizeof st);
     rc = hfv_mh2d_upload(h, (const double *const *)W, (const double *const *)U, NULL);
     if (rc == HFV_OK && store.on) rc = hfv_mh2d_set_pause(h, next_store_time(sp, &store), 0);
+    /* custom hooks: on the device when every one has a registered device
+     * form, else on the host around every step */
+    const int custom = has_custom_bc(bc, 1) && !(rc == HFV_OK && attach_device_bc(bc, h));
     if (rc == HFV_OK) rc = hfv_mh2d_start(h, NULL);
-    const int custom = has_custom_bc(bc, 1);
     if (rc == HFV_OK && (store.on || custom)) {
         /* batches of queued launches; the device pauses after each step
          * that is due a snapshot, which is written from the host arrays.
-         * With a custom boundary hook every step goes one at a time: the
-         * primitive state comes to the host, the reference's boundary fill
-         * runs there (hooks included, boundary.c:513-777) and the filled
-         * state goes back for the next step. */
+         * With a custom boundary hook on the host every step goes one at a
+         * time: the primitive state comes to the host, the reference's
+         * boundary fill runs there (hooks included, boundary.c:513-777) and
+         * the filled state goes back for the next step. */
         long long batch = 15, filled = 0;
         for (;;) {
             rc = hfv_mh2d_status(h, &st, NULL);

@@ -64,6 +64,7 @@ class Simulation:
         self.tx = nx + 2 * g
         self.ty = self.ny + 2 * g if self.dim == 2 else 1
         self.bc = lib.get_new_boundary_condition_param()
+        self._device_bc = []  # host hooks registered with a device form
         self._closed = False
 
     # -- helpers ---------------------------------------------------------
@@ -116,7 +117,12 @@ class Simulation:
                 dst[...] = a
 
     # -- API calls -------------------------------------------------------
-    def set_boundary(self, x_min, x_max, y_min=None, y_max=None, custom=None):
+    def set_boundary(self, x_min, x_max, y_min=None, y_max=None, custom=None, device=None):
+        """Boundary conditions by name; `custom` maps a hook field of
+        BoundaryConditionParam to a Python function (boundary.h:17-33).
+        `device` maps some of those fields to (device hook address, user
+        pointer) pairs, registered with hfv_register_device_bc_2d (this
+        library only) so that 2D runs fill those sides on the device."""
         bc = self.bc
         bc.boundary_condition_x_min = self._cstr(x_min)
         bc.boundary_condition_x_max = self._cstr(x_max)
@@ -126,7 +132,18 @@ class Simulation:
             cb = (abi.BC2DFunc if self.dim == 2 else abi.BC1DFunc)(fn)
             self._keep.append(cb)
             setattr(bc, name, cb)
+            if device and name in device:
+                self._register_device_bc(cb, *device[name])
         check(self.lib, self.lib.finalize_boundary_condition_param(C.byref(self.system), C.byref(bc)))
+
+    def _register_device_bc(self, host_cb, device_fn, user):
+        reg = self.lib.hfv_register_device_bc_2d
+        reg.restype = C.c_int
+        reg.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        host = C.cast(host_cb, C.c_void_p).value
+        if reg(host, device_fn, user) != 0:
+            raise HydroError(-1, "hfv_register_device_bc_2d failed (table full)")
+        self._device_bc.append(host)
 
     def apply_boundary(self):
         check(self.lib, self.lib.set_boundary_condition(C.byref(self.bc), C.byref(self.system)))
@@ -190,6 +207,9 @@ class Simulation:
 
     def close(self):
         if not self._closed:
+            for host in self._device_bc:
+                self.lib.hfv_register_device_bc_2d(host, None, None)
+            self._device_bc = []
             self.lib.free_system_memory(C.byref(self.system))
             self._closed = True
 

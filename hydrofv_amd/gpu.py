@@ -81,6 +81,8 @@ PROTOTYPES = {
     "hfv_mh2d_status": (C.c_int, [C.c_void_p, _P(RunStatus), C.c_void_p]),
     "hfv_mh2d_download": (C.c_int, [C.c_void_p, _P(dp), _P(dp), C.c_void_p]),
     "hfv_mh2d_fields": (C.c_int, [C.c_void_p, _P(C.c_void_p), _P(C.c_void_p), _P(C.c_void_p), _P(C.c_longlong)]),
+    "hfv_mh2d_set_device_bc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "hfv_register_device_bc_2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "hfv_mh1d_run": (C.c_int, [_P(MH1DConfig), _P(dp), _P(dp), dp, dp, dp, _P(RunStatus), C.c_void_p]),
     "hfv_probe_flux": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_longlong, dp, dp, dp,
                                  _P(C.c_int)]),

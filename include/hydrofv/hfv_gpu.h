@@ -106,6 +106,29 @@ int hfv_mh2d_download(hfv_mh2d *h, double *const W[4], double *const U[4], void 
  * of the device's own fill, its CFL maxima included. */
 int hfv_mh2d_primitive(hfv_mh2d *h, double *const W[4], void *stream);
 int hfv_mh2d_set_primitive(hfv_mh2d *h, const double *const W[4], void *stream);
+/* A custom boundary hook in device form: the reference's
+ * set_boundary_condition_cartesian_2d_func (boundary.h:25-33) given the
+ * primitive arrays in device memory, rows `pitch` doubles apart (pitch >=
+ * num_cells_x + 2 * num_ghost_cells_side), the hipStream_t of the run and the
+ * caller's pointer.  It enqueues its own kernels on that stream and returns,
+ * without synchronising the device or freeing the arrays.  It can be called
+ * again on an unchanged state (a paused or finished run), so its result
+ * should depend on the arrays alone. */
+#ifndef HFV_DEVICE_BC_2D_FUNC_DEFINED
+#define HFV_DEVICE_BC_2D_FUNC_DEFINED
+typedef void (*hfv_device_bc_2d_func)(double *density, double *velocity_x, double *velocity_y, double *pressure,
+                                      int num_ghost_cells_side, int num_cells_x, int num_cells_y, long long pitch,
+                                      void *stream, void *user);
+#endif
+/* Fill the custom side `side` (0..3 = x_min, x_max, y_min, y_max; cfg->bc of
+ * it must be 5, custom) with a device hook (the custom branches of
+ * boundary.c:513-777).  Once every custom side has one, each launch of
+ * hfv_mh2d_advance first refreshes the primitive buffer on the device: c2p
+ * of every cell (system.c:763-802), then the four sides in the reference's
+ * order, built-in fills and hooks, then the CFL maxima; the step reads that
+ * buffer.  hfv_mh2d_download returns the same filled state.  No graphs are
+ * used.  fn NULL removes the side's hook. */
+int hfv_mh2d_set_device_bc(hfv_mh2d *h, int side, hfv_device_bc_2d_func fn, void *user);
 /* Device addresses of the two conserved buffers and the primitive buffer. */
 int hfv_mh2d_fields(hfv_mh2d *h, double **U0, double **U1, double **W, long long *pitch);
 

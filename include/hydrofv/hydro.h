@@ -255,6 +255,23 @@ ErrorStatus hfv_godunov_first_order_1d_run(HFV_INTEGRATOR_ARGS, long long max_st
 ErrorStatus hfv_godunov_first_order_2d_run(HFV_INTEGRATOR_ARGS, long long max_steps, double fixed_dt);
 ErrorStatus hfv_random_choice_1d_run(HFV_INTEGRATOR_ARGS, long long max_steps, double fixed_dt);
 
+/* Extension (no reference counterpart): a device form of a custom 2D hook
+ * (boundary.h:25-33; the device hook type is the one of hfv_gpu.h).  While
+ * `host_hook` is registered, muscl_hancock_2d and godunov_first_order_2d
+ * fill the sides that name it with `device_hook` on the device state,
+ * instead of copying the state to the host around every step.  A run with a
+ * custom side whose hook is not registered keeps the host path.  Registering
+ * a hook again replaces its entry; device_hook NULL removes it.  At most 64
+ * entries; returns 0, or -1 for a NULL host_hook or a full table. */
+#ifndef HFV_DEVICE_BC_2D_FUNC_DEFINED
+#define HFV_DEVICE_BC_2D_FUNC_DEFINED
+typedef void (*hfv_device_bc_2d_func)(double *density, double *velocity_x, double *velocity_y, double *pressure,
+                                      int num_ghost_cells_side, int num_cells_x, int num_cells_y, long long pitch,
+                                      void *stream, void *user);
+#endif
+int hfv_register_device_bc_2d(set_boundary_condition_cartesian_2d_func host_hook,
+                              hfv_device_bc_2d_func device_hook, void *user);
+
 /* ---- storing.h:21-49 --------------------------------------------------- */
 StoringParam get_new_storing_param(void);                            /* storing.h:21 */
 ErrorStatus finalize_storing_param(StoringParam *storing_param);     /* storing.h:30 */

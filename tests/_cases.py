@@ -213,11 +213,28 @@ def copy_hook_2d(side):
     return hook
 
 
-def setup_cavity(lib, n, g=2):
-    """Lid-driven cavity: reflective walls, a custom moving lid on top."""
+def _counted(hook, calls):
+    if calls is None:
+        return hook
+
+    def wrapped(*args):
+        calls.append(1)
+        hook(*args)
+    return wrapped
+
+
+def setup_cavity(lib, n, g=2, device=False, calls=None, bcs=("reflective", "reflective", "reflective")):
+    """Lid-driven cavity: walls `bcs` (x_min, x_max, y_min), a custom moving
+    lid on top.  device: register the lid's device form as well (2D runs
+    then fill it on the device); calls: a list that grows by one per host
+    hook call."""
     s = driver.Simulation(lib, "cartesian_2d", 1.4, n, n, g)
-    s.set_boundary("reflective", "reflective", "reflective", "custom",
-                   custom={"set_boundary_condition_cartesian_2d_y_max": lid_hook()})
+    name = "set_boundary_condition_cartesian_2d_y_max"
+    dev = None
+    if device:
+        import _devhooks
+        dev = {name: _devhooks.lid()}
+    s.set_boundary(*bcs, "custom", custom={name: _counted(lid_hook(), calls)}, device=dev)
     one = np.ones((n, n))
     s.set_primitive([one, 0 * one, 0 * one, one / 1.4], interior_only=True)
     s.apply_boundary()
@@ -225,14 +242,19 @@ def setup_cavity(lib, n, g=2):
     return s
 
 
-def setup_all_custom(lib, nx, ny, g, seed):
-    """Random interior state with every side filled by a hook."""
+def setup_all_custom(lib, nx, ny, g, seed, device=False, calls=None, device_sides=(0, 1, 2, 3)):
+    """Random interior state with every side filled by a hook; device and
+    calls as for setup_cavity (device forms for `device_sides` only)."""
     s = driver.Simulation(lib, "cartesian_2d", 1.4, nx, ny, g)
+    names = ["set_boundary_condition_cartesian_2d_" + k for k in ("x_min", "x_max", "y_min", "y_max")]
+    hooks = [inflow_hook_2d(), copy_hook_2d(1), copy_hook_2d(2), copy_hook_2d(3)]
+    dev = None
+    if device:
+        import _devhooks
+        forms = [_devhooks.inflow(), _devhooks.copy(1), _devhooks.copy(2), _devhooks.copy(3)]
+        dev = {names[k]: forms[k] for k in device_sides}
     s.set_boundary("custom", "custom", "custom", "custom",
-                   custom={"set_boundary_condition_cartesian_2d_x_min": inflow_hook_2d(),
-                           "set_boundary_condition_cartesian_2d_x_max": copy_hook_2d(1),
-                           "set_boundary_condition_cartesian_2d_y_min": copy_hook_2d(2),
-                           "set_boundary_condition_cartesian_2d_y_max": copy_hook_2d(3)})
+                   custom={n: _counted(h, calls) for n, h in zip(names, hooks)}, device=dev)
     s.set_primitive(random_state_2d(nx, ny, seed), interior_only=True)
     s.apply_boundary()
     s.primitive_to_conserved()

@@ -167,3 +167,32 @@ def test_device_config_mirrors_match_the_header():
     assert out[5] == gpu.MH1DConfig.t_pause.offset
     assert out[6] == gpu.MH1DConfig.random_choice.offset
     assert out[7] == adjoint.Adj2DConfig.slope_limiter.offset
+
+
+def test_device_bc_registry():
+    """hfv_register_device_bc_2d: host-side table of device hook forms (no
+    device work): NULL host hook refused, re-registration replaces, NULL
+    device hook removes, at most 64 entries."""
+    import hydrofv_amd
+    lib = hydrofv_amd.load()
+    reg = lib.hfv_register_device_bc_2d
+    reg.restype = C.c_int
+    reg.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    base = 0x1000
+    assert reg(None, 0x10, None) == -1
+    try:
+        for k in range(64):
+            assert reg(base + 8 * k, 0x10, None) == 0
+        assert reg(base + 8 * 64, 0x10, None) == -1          # full
+        assert reg(base, 0x20, 0x30) == 0                     # replaces, no new entry
+        assert reg(base + 8, None, None) == 0                 # removes one
+        assert reg(base + 8 * 64, 0x10, None) == 0            # room again
+        assert reg(base + 8 * 65, 0x10, None) == -1
+        assert reg(base + 8 * 999, None, None) == 0           # removing an absent entry is harmless
+    finally:
+        for k in range(66):
+            reg(base + 8 * k, None, None)
+    for k in range(64):
+        assert reg(base + 8 * k, 0x10, None) == 0
+    for k in range(64):
+        reg(base + 8 * k, None, None)
