@@ -36,6 +36,7 @@ enum ErrKind : int {
     ERR_HLLC_SPEED = 2,     // riemann_solver_hllc.c:372-375
     ERR_EXACT_BRACKET = 3,  // riemann_solver_exact.c:702-707
     ERR_EXACT_BISECT = 4,   // riemann_solver_exact.c:745-754
+    ERR_PENDING = -1,       // internal: the face needs the iterative exact solve (DIRECT_ONLY)
 };
 
 // ---------------------------------------------------------------------------
@@ -683,7 +684,10 @@ struct Sample {
 // own state, or (rho, u*, p) on the star side of its rarefaction, whose fan
 // has no width.  The bits equal the full solve's (checked on the host build
 // against the full path, tests/test_adjoint_math.py).
-template <bool SHORTCUT = true>
+// DIRECT_ONLY: return rc = ERR_PENDING for every problem the shortcut
+// does not answer (the shortcut's answer is the same instructions as in the
+// full solve).
+template <bool SHORTCUT = true, bool DIRECT_ONLY = false>
 HFV_HD __forceinline__ Sample exact_sample(double gamma, double tol, double rl, double ul, double vl, double pl,
                                            double rr, double ur, double vr, double pr, double speed)
 {
@@ -706,6 +710,10 @@ HFV_HD __forceinline__ Sample exact_sample(double gamma, double tol, double rl, 
             }
             return S;
         }
+    }
+    if constexpr (DIRECT_ONLY) {
+        S.rc = ERR_PENDING;
+        return S;
     }
     if (rl <= 0.0 || rr <= 0.0 || ((al + ar) * 2.0 / (gamma + 1.0)) <= (ur - ul)) {
         S.w = vacuum(gamma, rl, ul, vl, pl, al, rr, ur, vr, pr, ar, speed);
@@ -737,12 +745,15 @@ HFV_HD __forceinline__ Sample exact_sample(double gamma, double tol, double rl, 
 // advected by the contact (v_L when 0 < u*, the "speed < u_star" side of
 // exact.c:431, else v_R); on the vacuum branch the vacuum sampler carries it.
 // At v = 0 the 2D flux equals the 1D flux bitwise.
-template <int NF>
+// DIRECT_ONLY: ERR_PENDING (f untouched) for a face whose states differ.
+template <int NF, bool DIRECT_ONLY = false>
 HFV_HD __forceinline__ int flux_exact(double gamma, double tol, const double* wl, const double* wr, double* f)
 {
     const double vl = (NF == 4) ? wl[2] : 0.0;
     const double vr = (NF == 4) ? wr[2] : 0.0;
-    const Sample S = exact_sample(gamma, tol, wl[0], wl[1], vl, wl[NF - 1], wr[0], wr[1], vr, wr[NF - 1], 0.0);
+    const Sample S = exact_sample<true, DIRECT_ONLY>(gamma, tol, wl[0], wl[1], vl, wl[NF - 1], wr[0], wr[1], vr,
+                                                     wr[NF - 1], 0.0);
+    if (DIRECT_ONLY && S.rc == ERR_PENDING) return ERR_PENDING;
     const double r = S.w.r, u = S.w.u, p = S.w.p;
     f[0] = r * u;
     f[1] = r * u * u + p;

@@ -32,6 +32,139 @@ __global__ void __launch_bounds__(LANES) k_mh2d_step(Grid2D G, RunParams rp, Fie
     strip_finish(r, out, lane);
 }
 
+// The exact-solver step with queued faces (Deferred, mh2d_strip.hpp): the
+// strip pass, then k_solve_faces, then k_update_deferred; k_defer_reset
+// clears the queues before the strip pass.
+template <int LIM, bool WMODE>
+__global__ void __launch_bounds__(LANES) k_mh2d_step_defer(Grid2D G, RunParams rp, Fields4 Ua, Fields4 Ub,
+                                                           Fields4 W, Ctrl* ctrl, int slot, Deferred D)
+{
+    const Ctrl* in = ctrl + slot;
+    Ctrl* out = ctrl + (slot + 1) % 3;
+    Ctrl* clear = ctrl + (slot + 2) % 3;
+    int dt_zero;
+    const StepPlan plan = plan_step(in, rp, &dt_zero);
+    const int lane = threadIdx.x;
+    if (blockIdx.x == 0 && lane == 0) publish(in, out, clear, plan, dt_zero);
+    if (!plan.run) return;
+    const Fields4 U = plan.cur ? Ub : Ua;
+    const Fields4 V = plan.cur ? Ua : Ub;
+    const int tile = xcd_remap(blockIdx.x, G.nsx * G.nsy);
+    const StripResult r =
+        strip_step<LIM, EXACT, WMODE, false, true>(G, plan.dt, U, V, W, tile % G.nsx, tile / G.nsx, lane, D);
+    strip_finish(r, out, lane);
+}
+
+__global__ void k_defer_reset(Deferred D)
+{
+    if (threadIdx.x == 0) {
+        D.count[0] = 0ull;
+        D.count[1] = 0ull;
+    }
+}
+
+// One queued face per lane: the full exact solve (riemann_solver_exact.c
+// via flux_exact), its flux stored at the face; a failure is reported with
+// the cell the strip pass would name.  The queue length was written by the
+// strip pass, which ran before this launch on the same stream.
+__global__ void __launch_bounds__(LANES) k_solve_faces(Grid2D G, Deferred D, Ctrl* ctrl, int slot)
+{
+    const long long n = (long long)min(D.count[0], (unsigned long long)D.face_cap);
+    for (long long i = (long long)blockIdx.x * LANES + threadIdx.x; i < n; i += (long long)gridDim.x * LANES) {
+        double l[4], r[4], o[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            l[q] = D.qw[q][i];
+            r[q] = D.qw[4 + q][i];
+        }
+        const int e = flux_exact<4>(G.gamma, G.tol, l, r, o);
+        const long long id = D.qid[i];
+        const long long k = id >> 1;
+        if (id & 1) {
+            // y face: (mass, my, mx, E) back to (mass, mx, my, E)
+            D.fy[0][k] = o[0];
+            D.fy[1][k] = o[2];
+            D.fy[2][k] = o[1];
+            D.fy[3][k] = o[3];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++) D.fx[q][k] = o[q];
+        }
+        if (e != ERR_NONE) {
+            const long long row = k / G.pitch, c = k % G.pitch;
+            report_error(ctrl + (slot + 1) % 3, e, row * G.tx + c - ((id & 1) ? 0 : 1));
+        }
+    }
+}
+
+// The deferred cells' updates, in the strip pass's operation order
+// (integrator_muscl_hancock.c:806-814, 1092-1100; gravity source_term.c:
+// 213-240), and their share of the CFL maxima.
+constexpr int UPD_THREADS = 256;
+
+__global__ void __launch_bounds__(UPD_THREADS) k_update_deferred(Grid2D G, RunParams rp, Fields4 Ua, Fields4 Ub,
+                                                                 Deferred D, Ctrl* ctrl, int slot)
+{
+    int dt_zero;
+    const StepPlan plan = plan_step(ctrl + slot, rp, &dt_zero);
+    if (!plan.run) return;
+    const Fields4 U = plan.cur ? Ub : Ua;
+    const Fields4 V = plan.cur ? Ua : Ub;
+    const double dt = plan.dt, gamma = G.gamma;
+    const long long n = (long long)min(D.count[1], (unsigned long long)D.cell_cap);
+    double smx = -INFINITY, smy = -INFINITY;
+    for (long long i = (long long)blockIdx.x * UPD_THREADS + threadIdx.x; i < n;
+         i += (long long)gridDim.x * UPD_THREADS) {
+        const long long e = D.cells[i];
+        const int kind = (int)(e & 7);
+        const long long k = e >> 3;
+        if (kind == DEF_INTERIOR) {
+            double u[4];
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                double x = U.f[f][k];
+                x += D.fx[f][k] * dt * G.area_x;
+                x -= D.fx[f][k + 1] * dt * G.area_x;
+                x += D.fy[f][k - G.pitch] * dt * G.area_y;
+                x -= D.fy[f][k] * dt * G.area_y;
+                u[f] = x;
+            }
+            if (G.gravity != 0.0) {
+                const double pre = -dt * G.gravity;
+                const double kmy = pre * u[0];
+                const double k1 = pre * u[2];
+                const double k2 = pre * (u[2] + 0.5 * kmy);
+                const double k3 = k2;
+                const double k4 = pre * (u[2] + kmy);
+                u[2] += kmy;
+                u[3] += (k1 + 2.0 * k2 + 2.0 * k3 + k4) / 6.0;
+            }
+#pragma unroll
+            for (int f = 0; f < 4; f++) V.f[f][k] = u[f];
+            double w[4];
+            c2p_2d(gamma, G.volume, u[0], u[1], u[2], u[3], w);
+            const double a = sound_speed(gamma, w[0], w[3]);
+            const double sx = a + fabs(w[1]);
+            const double sy = a + fabs(w[2]);
+            if (sx > smx) smx = sx;
+            if (sy > smy) smy = sy;
+        } else if (kind == DEF_GHOST_XLO) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] - D.fx[f][k + 1] * dt * G.area_x;
+        } else if (kind == DEF_GHOST_XHI) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] + D.fx[f][k] * dt * G.area_x;
+        } else if (kind == DEF_GHOST_YLO) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] - D.fy[f][k] * dt * G.area_y;
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] + D.fy[f][k - G.pitch] * dt * G.area_y;
+        }
+    }
+    block_smax_to<UPD_THREADS>(ctrl + (slot + 1) % 3, smx, smy);
+}
+
 // CFL maxima of a primitive state held in memory (the first step's dt):
 // workgroups stride over the interior rows, reduce in registers and LDS, and
 // add one atomic per key (the comparisons of utils.c:75-92 skip NaN).
@@ -205,6 +338,8 @@ struct hfv_mh2d {
     void* dev_user[4];
     long long w_filled;    // launch count at which the primitive buffer was last filled (device hooks)
     double* stage;         // one contiguous field for host copies (allocated at first use)
+    Deferred D;            // exact solver: queued faces and deferred cells
+    double* defer_slab;
 };
 
 namespace {
@@ -260,6 +395,30 @@ void launch_step_t(hfv_mh2d* h, bool wmode, hipStream_t s)
     h->launches++;
 }
 
+// exact solver: queue reset, strip pass, queued faces, deferred cells
+template <int LIM>
+void launch_exact(hfv_mh2d* h, bool wmode, hipStream_t s)
+{
+    const int slot = (int)(h->launches % 3);
+    const Grid2D& G = h->G;
+    const long long cells = (long long)G.tx * G.ty;
+    const unsigned solve_blocks = (unsigned)std::min<long long>(4096, (2 * cells + LANES - 1) / LANES);
+    const unsigned update_blocks = (unsigned)std::min<long long>(1024, (cells + UPD_THREADS - 1) / UPD_THREADS);
+    hipLaunchKernelGGL(k_defer_reset, dim3(1), dim3(64), 0, s, h->D);
+    const dim3 grid(G.nsx * G.nsy), block(LANES);
+    if (wmode) {
+        hipLaunchKernelGGL((k_mh2d_step_defer<LIM, true>), grid, block, 0, s, G, h->rp, h->U[0], h->U[1], h->W,
+                           h->ctrl, slot, h->D);
+    } else {
+        hipLaunchKernelGGL((k_mh2d_step_defer<LIM, false>), grid, block, 0, s, G, h->rp, h->U[0], h->U[1], h->W,
+                           h->ctrl, slot, h->D);
+    }
+    hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), dim3(LANES), 0, s, G, h->D, h->ctrl, slot);
+    hipLaunchKernelGGL(k_update_deferred, dim3(update_blocks), dim3(UPD_THREADS), 0, s, G, h->rp, h->U[0], h->U[1],
+                       h->D, h->ctrl, slot);
+    h->launches++;
+}
+
 template <int LIM>
 void launch_hllc(hfv_mh2d* h, bool wmode, hipStream_t s)
 {
@@ -272,15 +431,15 @@ void launch_step(hfv_mh2d* h, bool wmode, hipStream_t s)
     const int lim = h->cfg.slope_limiter, sol = h->cfg.riemann_solver;
     if (h->cfg.first_order) {
         if (sol == HLLC) launch_hllc<FIRST_ORDER>(h, wmode, s);
-        else launch_step_t<FIRST_ORDER, EXACT, false>(h, wmode, s);
+        else launch_exact<FIRST_ORDER>(h, wmode, s);
     } else if (sol == HLLC) {
         if (lim == MINMOD) launch_hllc<MINMOD>(h, wmode, s);
         else if (lim == VAN_LEER) launch_hllc<VAN_LEER>(h, wmode, s);
         else launch_hllc<MC>(h, wmode, s);
     } else {
-        if (lim == MINMOD) launch_step_t<MINMOD, EXACT, false>(h, wmode, s);
-        else if (lim == VAN_LEER) launch_step_t<VAN_LEER, EXACT, false>(h, wmode, s);
-        else launch_step_t<MC, EXACT, false>(h, wmode, s);
+        if (lim == MINMOD) launch_exact<MINMOD>(h, wmode, s);
+        else if (lim == VAN_LEER) launch_exact<VAN_LEER>(h, wmode, s);
+        else launch_exact<MC>(h, wmode, s);
     }
 }
 
@@ -398,11 +557,38 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
         h->W.f[f] = h->slab + field * (8 + f);
     }
     h->ctrl = reinterpret_cast<Ctrl*>(h->slab + field * 12);
+    if (cfg->riemann_solver == EXACT) {
+        // every face can be queued (each has one owning lane) and every cell
+        // deferred (each has one updating lane)
+        Deferred& D = h->D;
+        D.cell_cap = (long long)G.tx * G.ty;
+        D.face_cap = 2 * D.cell_cap;
+        const size_t dbytes = (field * 8 + (size_t)D.face_cap * 9 + (size_t)D.cell_cap + 2) * sizeof(double);
+        if (hipMalloc(&h->defer_slab, dbytes) != hipSuccess) {
+            (void)hipFree(h->slab);
+            delete h;
+            return HFV_ENOMEM;
+        }
+        double* p = h->defer_slab;
+        for (int f = 0; f < 4; f++) {
+            D.fx[f] = p + field * f;
+            D.fy[f] = p + field * (4 + f);
+        }
+        p += field * 8;
+        for (int q = 0; q < 8; q++) D.qw[q] = p + (size_t)D.face_cap * q;
+        p += (size_t)D.face_cap * 8;
+        D.qid = reinterpret_cast<long long*>(p);
+        p += D.face_cap;
+        D.cells = reinterpret_cast<long long*>(p);
+        p += D.cell_cap;
+        D.count = reinterpret_cast<unsigned long long*>(p);
+    }
     h->launches = 0;
     h->graph = nullptr;
     h->graph_len = 0;
     h->w_filled = -1;
     if (hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess) {
+        if (h->defer_slab) (void)hipFree(h->defer_slab);
         (void)hipFree(h->slab);
         delete h;
         return HFV_EDEVICE;
@@ -418,6 +604,7 @@ void hfv_mh2d_destroy(hfv_mh2d* h)
     (void)hipStreamSynchronize(h->own);
     (void)hipStreamDestroy(h->own);
     if (h->stage) (void)hipFree(h->stage);
+    if (h->defer_slab) (void)hipFree(h->defer_slab);
     (void)hipFree(h->slab);
     delete h;
 }

@@ -192,6 +192,93 @@ __device__ __forceinline__ void trrs_powers(double gamma, double tol, HllcFace& 
 __device__ __forceinline__ double shfl_from_left(double v) { return __shfl_up(v, 1, LANES); }
 __device__ __forceinline__ double shfl_from_right(double v) { return __shfl_down(v, 1, LANES); }
 
+// DEFER (exact solver): a face whose two states differ (its Riemann problem
+// needs the pressure iteration, or the vacuum sampler) is not solved by the
+// lane that reconstructs it but queued, with
+// its two states, for k_solve_faces, which spreads the queue over the whole
+// chip; a wavefront of a blast region would otherwise solve its rows' faces
+// one after another, with its lanes diverging, while the rest of the chip
+// idles.  A cell next to a queued face is deferred too: its other faces'
+// fluxes go to fx/fy and k_update_deferred applies the update, in the
+// reference's order, once every flux is known.  Every other face and cell
+// is done in the strip as usual.
+struct Deferred {
+    double* fx[4];              // flux through the face left of cell (q, c), at q * pitch + c
+    double* fy[4];              // flux through the face above cell (q, c), at q * pitch + c
+    double* qw[8];              // queued faces: left then right state, in the solver's order
+    long long* qid;             // queued faces: (q * pitch + c) * 2 + (1: face above, 0: face left)
+    long long* cells;           // deferred cells: (q * pitch + c) * 8 + DeferKind
+    unsigned long long* count;  // [0] queued faces, [1] deferred cells
+    long long face_cap, cell_cap;
+};
+
+enum DeferKind : int { DEF_INTERIOR = 0, DEF_GHOST_XLO = 1, DEF_GHOST_XHI = 2, DEF_GHOST_YLO = 3, DEF_GHOST_YHI = 4 };
+
+// Wave-aggregated append: one atomic per wavefront; each lane with `take`
+// gets its own slot (-1 for the others).  Every lane must call this.
+__device__ __forceinline__ long long wave_append(unsigned long long* counter, bool take, int lane)
+{
+    const unsigned long long m = __ballot(take);
+    if (m == 0ull) return -1;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0ull;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, LANES);
+    return take ? (long long)(base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull))) : -1;
+}
+
+// The exact flux of a face in (mass, mx, my, E) order when its states are
+// equal (exact_sample's shortcut); true (f zero) when it must be queued.
+// YFACE: the solver sees (rho, v, u, p), as in yflux.
+template <bool YFACE>
+__device__ __forceinline__ bool direct_face(double gamma, double tol, const double* l, const double* r, double* f)
+{
+    double ls[4], rs[4], o[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int t = YFACE ? ((q == 1) ? 2 : (q == 2) ? 1 : q) : q;
+        ls[q] = l[t];
+        rs[q] = r[t];
+    }
+    if (flux_exact<4, true>(gamma, tol, ls, rs, o) == ERR_PENDING) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) f[q] = 0.0;
+        return true;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) f[q] = o[YFACE ? ((q == 1) ? 2 : (q == 2) ? 1 : q) : q];
+    return false;
+}
+
+// Queue the owned faces that need the iteration (states in the solver's order).
+template <bool YFACE>
+__device__ __forceinline__ void queue_face(const Deferred& D, bool take, long long k, const double* l,
+                                           const double* r, int lane)
+{
+    const long long i = wave_append(&D.count[0], take, lane);
+    if (i >= 0 && i < D.face_cap) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int t = YFACE ? ((q == 1) ? 2 : (q == 2) ? 1 : q) : q;
+            D.qw[q][i] = l[t];
+            D.qw[4 + q][i] = r[t];
+        }
+        D.qid[i] = k * 2 + (YFACE ? 1 : 0);
+    }
+}
+
+__device__ __forceinline__ void defer_cell(const Deferred& D, bool take, long long k, int kind, int lane)
+{
+    const long long i = wave_append(&D.count[1], take, lane);
+    if (i >= 0 && i < D.cell_cap) D.cells[i] = k * 8 + kind;
+}
+
+__device__ __forceinline__ void store_flux(double* const* F, long long k, const double* f)
+{
+#pragma unroll
+    for (int q = 0; q < 4; q++) F[q][k] = f[q];
+}
+
 struct StripResult {
     double smx, smy;     // CFL maxima of the strip's updated cells
     int err;             // ErrKind of a failed flux (0: none)
@@ -204,10 +291,12 @@ struct StripResult {
 // integrator_godunov_first_order.c:362-497, whose faces take the cells' own
 // primitive states; its sweeps, update order and source term are those of
 // the MUSCL-Hancock step, and it needs one ghost layer, not two.
-template <int LIM, int SOLVER, bool WMODE, bool BRANCHY>
+template <int LIM, int SOLVER, bool WMODE, bool BRANCHY, bool DEFER = false>
 __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, const Fields4& U, const Fields4& V,
-                                                  const Fields4& W, int sx, int sy, int lane)
+                                                  const Fields4& W, int sx, int sy, int lane,
+                                                  const Deferred& D = Deferred())
 {
+    static_assert(!DEFER || SOLVER == EXACT, "only exact-solver faces are queued");
     const double cx = dt * G.area_x / G.volume;  // dt*A/V, integrator_muscl_hancock.c:636
     const double cy = dt * G.area_y / G.volume;  // :921
 
@@ -220,6 +309,14 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     const bool ghost_lo_col = (sx == 0) && (col == g - 1);               // x_min face flux only
     const bool ghost_hi_col = (sx == G.nsx - 1) && (col == g + G.nx);    // x_max face flux only
     const double gamma = G.gamma, tol = G.tol;
+    // DEFER: the lane's x face (left of col) is used by this wavefront when
+    // its own cell or the cell on its left is updated here, and queued by
+    // the lane whose own cell uses it (one owner per face)
+    const bool left_used = (lane >= 1) && (((lane - 1 >= HALO) && (lane - 1 < HALO + STRIP_COLS) &&
+                                            (col - 1 < g + G.nx)) || ((sx == 0) && (col == g)));
+    const bool need_x = out_col || ghost_hi_col || left_used;
+    const bool own_x = out_col || ghost_hi_col;
+    bool hyb = false;  // DEFER: the flux below the current row is queued
 
     int err = ERR_NONE;
     long long err_cell = -1;
@@ -238,8 +335,13 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             wb[f] = wa[f];  // unused
             yt[f] = wa[f];
         }
-        const int e = yflux<SOLVER>(gamma, tol, wm1, wa, fyb);
-        if (e && out_col) { err = e; err_cell = (long long)(y0 - 1) * G.tx + col; }
+        if constexpr (DEFER) {
+            hyb = out_col && direct_face<true>(gamma, tol, wm1, wa, fyb);
+            queue_face<true>(D, hyb && y0 == g, (long long)(y0 - 1) * G.pitch + col, wm1, wa, lane);
+        } else {
+            const int e = yflux<SOLVER>(gamma, tol, wm1, wa, fyb);
+            if (e && out_col) { err = e; err_cell = (long long)(y0 - 1) * G.tx + col; }
+        }
     } else {
         double wm2[4], wm1[4];
         load_prim<WMODE>(G, U, W, y0 - 2, col, wm2);
@@ -249,10 +351,19 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         double lo[4], hi_m1[4], lo0[4];
         ypredict<LIM>(wm2, wm1, wa, cy, gamma, lo, hi_m1);   // row y0-1: top face state
         ypredict<LIM>(wm1, wa, wb, cy, gamma, lo0, yt);      // row y0: bottom and top face states
-        const int e = yflux<SOLVER>(gamma, tol, hi_m1, lo0, fyb);
-        if (e && out_col) { err = e; err_cell = (long long)(y0 - 1) * G.tx + col; }
+        if constexpr (DEFER) {
+            hyb = out_col && direct_face<true>(gamma, tol, hi_m1, lo0, fyb);
+            queue_face<true>(D, hyb && y0 == g, (long long)(y0 - 1) * G.pitch + col, hi_m1, lo0, lane);
+        } else {
+            const int e = yflux<SOLVER>(gamma, tol, hi_m1, lo0, fyb);
+            if (e && out_col) { err = e; err_cell = (long long)(y0 - 1) * G.tx + col; }
+        }
     }
-    if (y0 == g && out_col) {
+    if constexpr (DEFER) {
+        // the bottom ghost row's update waits for a queued y_min face
+        defer_cell(D, y0 == g && out_col && hyb, (long long)(g - 1) * G.pitch + col, DEF_GHOST_YLO, lane);
+    }
+    if (y0 == g && out_col && !hyb) {
         // bottom ghost row receives the y_min face flux (1092-1095)
         const long long k = (long long)(g - 1) * G.pitch + col;
         V.f[0][k] = U.f[0][k] - fyb[0] * dt * G.area_y;
@@ -293,6 +404,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         for (int f = 0; f < 4; f++) hl[f] = shfl_from_left(xhi[f]);
         double fyt[4], fxl[4];
         int ey, ex;
+        bool hy = false, hx = false;  // DEFER: this row's top face / the lane's x face is queued
         if constexpr (SOLVER == HLLC) {
             // both faces' fluxes, with the wavefront's two-rarefaction
             // powers taken together
@@ -335,6 +447,14 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             fyt[3] = o[0][3];
 #pragma unroll
             for (int f = 0; f < 4; f++) fxl[f] = o[1][f];
+        } else if constexpr (DEFER) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) fyt[f] = fxl[f] = 0.0;
+            hy = out_col && direct_face<true>(gamma, tol, yt, ylo, fyt);
+            hx = need_x && direct_face<false>(gamma, tol, hl, xlo, fxl);
+            queue_face<true>(D, hy, (long long)r * G.pitch + col, yt, ylo, lane);
+            queue_face<false>(D, hx && own_x, (long long)r * G.pitch + col, hl, xlo, lane);
+            ey = ex = ERR_NONE;
         } else {
             ey = yflux<SOLVER>(gamma, tol, yt, ylo, fyt);
             ex = flux<SOLVER, 4>(gamma, tol, hl, xlo, fxl);
@@ -346,7 +466,31 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         for (int f = 0; f < 4; f++) fxr[f] = shfl_from_right(fxl[f]);
 
         const long long k = (long long)r * G.pitch + col;
-        if (out_col) {
+        bool deferred = false;
+        if constexpr (DEFER) {
+            const bool hxr = __shfl_down((int)hx, 1, LANES) != 0;
+            int kind = DEF_INTERIOR;
+            if (out_col) {
+                deferred = hx || hxr || hyb || hy;
+                if (deferred) {
+                    // the cell's direct fluxes, for k_update_deferred
+                    if (!hx) store_flux(D.fx, k, fxl);
+                    if (!hxr) store_flux(D.fx, k + 1, fxr);
+                    if (!hyb) store_flux(D.fy, k - G.pitch, fyb);
+                    if (!hy) store_flux(D.fy, k, fyt);
+                }
+            } else if (ghost_lo_col) {
+                deferred = hxr;
+                kind = DEF_GHOST_XLO;
+            } else if (ghost_hi_col) {
+                deferred = hx;
+                kind = DEF_GHOST_XHI;
+            }
+            defer_cell(D, deferred, k, kind, lane);
+        }
+        if (deferred) {
+            // updated by k_update_deferred
+        } else if (out_col) {
             double u[4];
 #pragma unroll
             for (int f = 0; f < 4; f++) {
@@ -393,8 +537,13 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             yt[f] = yhi[f];
             fyb[f] = fyt[f];
         }
+        hyb = hy;
     }
-    if (y1 == g + G.ny && out_col) {
+    if constexpr (DEFER) {
+        // the top ghost row's update waits for a queued y_max face
+        defer_cell(D, y1 == g + G.ny && out_col && hyb, (long long)(g + G.ny) * G.pitch + col, DEF_GHOST_YHI, lane);
+    }
+    if (y1 == g + G.ny && out_col && !hyb) {
         // top ghost row receives the y_max face flux (1097-1100)
         const long long k = (long long)(g + G.ny) * G.pitch + col;
 #pragma unroll

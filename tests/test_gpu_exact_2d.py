@@ -1,0 +1,82 @@
+"""The 2D exact-solver step with queued faces (k_mh2d_step_defer,
+k_solve_faces, k_update_deferred): faces whose Riemann problem needs the
+pressure iteration are solved by a chip-wide pass and the cells beside them
+updated afterwards, in the reference's update order.  Random states queue
+nearly every face, so every deferral kind (interior cells, the four ghost
+strips) and every strip layout is exercised; a blast queues only the faces
+near the shock.  Compared with the CPU oracle running the same exact solver
+(riemann_solver_exact.c, extended to 2D) within 1e-10 relative: the device
+computes pow() with pow_pos, which can differ from glibc in the last place."""
+import numpy as np
+import pytest
+
+import _cases as K
+from hydrofv_amd import driver
+from test_gpu_parity_2d import compare, interior_and_bc_mask, run_pair
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+SHAPES = [(13, 7, 2), (61, 33, 2), (130, 70, 2), (64, 40, 3)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}g{s[2]}")
+@pytest.mark.parametrize("bcs", list(K.BC_SETS), ids=str)
+def test_random_states_queue_every_face(gpulib, shape, bcs):
+    """130 columns make three strips (the last partly filled), 70 rows
+    several row strips; g = 3 leaves a ghost layer no face touches."""
+    nx, ny, g = shape
+    setup = lambda lib: K.setup_random_2d(lib, nx, ny, g, K.BC_SETS[bcs], 900 + nx + ny + g)
+    sim, W, U, Wo, Uo = run_pair(gpulib, setup, "van_leer", solver="riemann_solver_exact", steps=6, cfl=0.5,
+                                 tol=1e-8)
+    compare(f"exact {shape} {bcs}", W, U, Wo, Uo, tol=TOL)
+    sim.close()
+
+
+@pytest.mark.parametrize("limiter", K.LIMITERS)
+def test_limiters(gpulib, limiter):
+    setup = lambda lib: K.setup_random_2d(lib, 70, 45, 2, K.BC_SETS["mixed"], 41)
+    sim, W, U, Wo, Uo = run_pair(gpulib, setup, limiter, solver="riemann_solver_exact", steps=8, cfl=0.5)
+    compare(f"exact {limiter}", W, U, Wo, Uo, tol=TOL)
+
+
+def test_first_order(gpulib):
+    sim = K.setup_random_2d(gpulib, 70, 45, 1, K.BC_SETS["mixed"], 43)
+    grid = K.oracle_grid(sim, "minmod", "riemann_solver_exact")
+    res = sim.launch("godunov_first_order_2d", "riemann_solver_exact", "minmod", 0.5, tf=np.inf, max_steps=9)
+    r = grid.run(0.5, max_steps=9, first_order=True)
+    assert r.error == 0 and res.num_steps == r.num_steps == 9
+    W, U = K.state(sim)
+    compare("exact first order", W, U, grid.W, grid.U, tol=TOL)
+
+
+def test_gravity(gpulib):
+    """The deferred cells take the gravity source like the others."""
+    setup = lambda lib: K.setup_random_2d(lib, 40, 30, 2, K.BC_SETS["reflective"], 45, gravity=0.9)
+    sim, W, U, Wo, Uo = run_pair(gpulib, setup, "monotonized_central", solver="riemann_solver_exact", steps=8,
+                                 cfl=0.5)
+    compare("exact gravity", W, U, Wo, Uo, tol=TOL)
+
+
+def test_sedov_blast_queues_the_shock(gpulib):
+    """A blast: the undisturbed faces are solved in the strip pass, the
+    shock's in the queue; run past the first 10 steps of the CFL ramp."""
+    sim, W, U, Wo, Uo = run_pair(gpulib, lambda lib: K.setup_sedov(lib, 96), "van_leer",
+                                 solver="riemann_solver_exact", steps=40, cfl=0.5, tol=1e-8)
+    compare("sedov 96", W, U, Wo, Uo, tol=TOL, mask=interior_and_bc_mask(sim))
+
+
+def test_failed_solve_is_reported(gpulib):
+    """A NaN pressure makes the queued solve fail (riemann_solver_exact.c:
+    745-754: bisection does not converge); the run stops with that error."""
+    sim = driver.Simulation(gpulib, "cartesian_2d", 1.4, 16, 12, 2)
+    sim.set_boundary("transmissive", "transmissive", "transmissive", "transmissive")
+    W = K.random_state_2d(16, 12, 47)
+    W[3][5, 7] = np.nan
+    sim.set_primitive(W, interior_only=True)
+    sim.apply_boundary()
+    sim.primitive_to_conserved()
+    with pytest.raises(driver.HydroError) as e:
+        sim.launch("muscl_hancock_2d", "riemann_solver_exact", "minmod", 0.5, tf=1.0)
+    assert e.value.code == 1
+    assert "Bisection method did not converge" in e.value.text
