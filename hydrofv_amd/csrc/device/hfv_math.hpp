@@ -511,15 +511,26 @@ HFV_HD __forceinline__ double wave_f(double gamma, double rho, double p, double 
     return 2.0 * a * (pow_pos((ps / p), (gamma - 1.0) / (2.0 * gamma)) - 1.0) / (gamma - 1.0);
 }
 
-// exact.c:539-565 -- the shock branch's (B - p*) is kept as written (553)
-HFV_HD __forceinline__ double wave_df(double gamma, double rho, double p, double a, double ps)
+// wave_f and its derivative (exact.c:539-565) at one pressure, for the
+// Newton step.  The derivative's shock branch keeps the (B - p*) of
+// exact.c:553 as written.  The shock branch's square root serves both
+// (ps + B and B + ps are the same sum), and so does the rarefaction
+// branch's log(ps / p) (pow_log); each result has the bits of evaluating
+// the reference's two functions separately.
+HFV_HD __forceinline__ void wave_f_df(double gamma, double rho, double p, double a, double ps, double* f, double* df)
 {
     if (ps > p) {
         const double A = coeff_A(gamma, rho);
         const double B = coeff_B(gamma, p);
-        return (1.0 - 0.5 * (ps - p) / (B - ps)) * sqrt(A / (B + ps));
+        const double s = sqrt(A / (ps + B));
+        *f = (ps - p) * s;
+        *df = (1.0 - 0.5 * (ps - p) / (B - ps)) * s;
+        return;
     }
-    return pow_pos(ps / p, -0.5 * (gamma + 1.0) / gamma) / (p * a);
+    const double x = ps / p;
+    const DD l = pow_log(x);
+    *f = 2.0 * a * (pow_with_log(x, (gamma - 1.0) / (2.0 * gamma), l) - 1.0) / (gamma - 1.0);
+    *df = pow_with_log(x, -0.5 * (gamma + 1.0) / gamma, l) / (p * a);
 }
 
 // The side states are (rho, u, p) in the exact solver (3-component order).
@@ -542,13 +553,20 @@ HFV_HD __forceinline__ PStar solve_p_star(double gamma, Side L, Side R, double t
     const double pg = guess_p(gamma, L.r, L.u, L.p, L.a, R.r, R.u, R.p, R.a, tol);
     bool bracketed = false;
     double p_hi = pg;
-    double f_hi = pressure_fn(gamma, L, R, p_hi);
+    // f and f' at the guess, the first iterate (f_hi is f there)
+    double fl, dfl, fr, dfr;
+    wave_f_df(gamma, L.r, L.p, L.a, pg, &fl, &dfl);
+    wave_f_df(gamma, R.r, R.p, R.a, pg, &fr, &dfr);
+    double f_hi = fl + fr + (R.u - L.u);
     double p_lo = 0.0, f_lo = 0.0;
     double p0 = pg;
     for (int it = 0; it < 100; it++) {
-        // the first iterate is the guess, where f_hi was just evaluated
-        const double f = (it == 0) ? f_hi : pressure_fn(gamma, L, R, p0);
-        const double df = wave_df(gamma, L.r, L.p, L.a, p0) + wave_df(gamma, R.r, R.p, R.a, p0);
+        if (it > 0) {
+            wave_f_df(gamma, L.r, L.p, L.a, p0, &fl, &dfl);
+            wave_f_df(gamma, R.r, R.p, R.a, p0, &fr, &dfr);
+        }
+        const double f = fl + fr + (R.u - L.u);
+        const double df = dfl + dfr;
         if (!bracketed && f * f_hi < 0.0) {
             bracketed = true;
             f_lo = f;

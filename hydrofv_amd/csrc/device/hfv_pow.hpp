@@ -134,12 +134,20 @@ HFV_HD __forceinline__ double exp_dd(DD w)
     return ldexp(s.hi + low, e);
 }
 
-// x^y for all x, y (C99 pow semantics; accuracy as stated at the top)
-HFV_HD __forceinline__ double pow_pos(double x, double y)
+// The logarithm pow_pos takes of x: log |x| for finite nonzero x, else 0.
+HFV_HD __forceinline__ DD pow_log(double x)
 {
     const double ax = fabs(x);
     const bool regular = (ax > 0.0) & (ax < INFINITY);  // false for 0, inf, NaN
-    const DD l = log_dd(regular ? ax : 1.0);
+    return log_dd(regular ? ax : 1.0);
+}
+
+// x^y given l = pow_log(x): the rest of pow_pos, so that powers of one base
+// share its logarithm with the same bits as separate pow_pos calls.
+HFV_HD __forceinline__ double pow_with_log(double x, double y, DD l)
+{
+    const double ax = fabs(x);
+    const bool regular = (ax > 0.0) & (ax < INFINITY);
     const double wh = y * l.hi;
     const double wl = fma(y, l.hi, -wh) + y * l.lo;
     // beyond these x^y rounds to infinity or to zero
@@ -164,5 +172,8 @@ HFV_HD __forceinline__ double pow_pos(double x, double y)
     r = ((y == 0.0) | (x == 1.0) | ((ax == 1.0) & (fabs(y) == INFINITY))) ? 1.0 : r;
     return r;
 }
+
+// x^y for all x, y (C99 pow semantics; accuracy as stated at the top)
+HFV_HD __forceinline__ double pow_pos(double x, double y) { return pow_with_log(x, y, pow_log(x)); }
 
 }  // namespace hfv
