@@ -548,38 +548,77 @@ struct PStar {
     int rc;
 };
 
-HFV_HD __forceinline__ PStar solve_p_star(double gamma, Side L, Side R, double tol)
+// The pressure iteration of exact.c:604-763 in resumable pieces: the
+// Newton loop can stop after any iteration and continue later from the
+// state it leaves, with the same operations on the same values as one
+// uninterrupted loop.
+struct NewtonState {
+    double p0;         // current iterate
+    double p_hi, f_hi; // the guess and f there (the loop's reference sign)
+    double df0;        // f' at the guess (the first iteration's)
+    double p_lo, f_lo; // the first iterate where f changed sign
+    int it;            // next iteration
+    int bracketed;
+};
+
+HFV_HD __forceinline__ NewtonState newton_begin(double gamma, Side L, Side R, double tol)
 {
     const double pg = guess_p(gamma, L.r, L.u, L.p, L.a, R.r, R.u, R.p, R.a, tol);
-    bool bracketed = false;
-    double p_hi = pg;
-    // f and f' at the guess, the first iterate (f_hi is f there)
     double fl, dfl, fr, dfr;
     wave_f_df(gamma, L.r, L.p, L.a, pg, &fl, &dfl);
     wave_f_df(gamma, R.r, R.p, R.a, pg, &fr, &dfr);
-    double f_hi = fl + fr + (R.u - L.u);
-    double p_lo = 0.0, f_lo = 0.0;
-    double p0 = pg;
-    for (int it = 0; it < 100; it++) {
-        if (it > 0) {
-            wave_f_df(gamma, L.r, L.p, L.a, p0, &fl, &dfl);
-            wave_f_df(gamma, R.r, R.p, R.a, p0, &fr, &dfr);
+    NewtonState s;
+    s.p0 = pg;
+    s.p_hi = pg;
+    s.f_hi = fl + fr + (R.u - L.u);
+    s.df0 = dfl + dfr;
+    s.p_lo = 0.0;
+    s.f_lo = 0.0;
+    s.it = 0;
+    s.bracketed = 0;
+    return s;
+}
+
+// Newton iterations s.it .. it_end - 1 (it_end <= 100).  Returns 1 with
+// *ps when the iterate converged, -1 when the loop has ended without
+// converging (a negative iterate, or 100 iterations), 0 when it stopped at
+// it_end.
+HFV_HD __forceinline__ int newton_run(double gamma, Side L, Side R, double tol, NewtonState& s, int it_end,
+                                      double* ps)
+{
+    for (; s.it < it_end; s.it++) {
+        double f, df;
+        if (s.it == 0) {
+            f = s.f_hi;
+            df = s.df0;
+        } else {
+            double fl, dfl, fr, dfr;
+            wave_f_df(gamma, L.r, L.p, L.a, s.p0, &fl, &dfl);
+            wave_f_df(gamma, R.r, R.p, R.a, s.p0, &fr, &dfr);
+            f = fl + fr + (R.u - L.u);
+            df = dfl + dfr;
         }
-        const double f = fl + fr + (R.u - L.u);
-        const double df = dfl + dfr;
-        if (!bracketed && f * f_hi < 0.0) {
-            bracketed = true;
-            f_lo = f;
-            p_lo = p0;
+        if (!s.bracketed && f * s.f_hi < 0.0) {
+            s.bracketed = 1;
+            s.f_lo = f;
+            s.p_lo = s.p0;
         }
-        const double p1 = p0 - f / df;
-        if (p1 < 0.0) break;
-        if (2.0 * fabs(p1 - p0) / (p1 + p0) < tol) {
-            return PStar{p1, ERR_NONE};
+        const double p1 = s.p0 - f / df;
+        if (p1 < 0.0) return -1;
+        if (2.0 * fabs(p1 - s.p0) / (p1 + s.p0) < tol) {
+            *ps = p1;
+            return 1;
         }
-        p0 = p1;
+        s.p0 = p1;
     }
-    if (!bracketed) {
+    return (s.it >= 100) ? -1 : 0;
+}
+
+// After a Newton loop that did not converge: bracketing and bisection.
+HFV_HD __forceinline__ PStar newton_finish(double gamma, Side L, Side R, double tol, const NewtonState& s)
+{
+    double p_lo = s.p_lo, f_lo = s.f_lo, p_hi = s.p_hi, f_hi = s.f_hi;
+    if (!s.bracketed) {
         p_lo = 0.0;
         f_lo = pressure_fn(gamma, L, R, p_lo);
         if (f_lo * f_hi >= 0.0) {
@@ -618,6 +657,14 @@ HFV_HD __forceinline__ PStar solve_p_star(double gamma, Side L, Side R, double t
         count++;
         if (count > 5000) return PStar{0.0, ERR_EXACT_BISECT};
     }
+}
+
+HFV_HD __forceinline__ PStar solve_p_star(double gamma, Side L, Side R, double tol)
+{
+    NewtonState s = newton_begin(gamma, L, R, tol);
+    double ps;
+    if (newton_run(gamma, L, R, tol, s, 100, &ps) > 0) return PStar{ps, ERR_NONE};
+    return newton_finish(gamma, L, R, tol, s);
 }
 
 // exact.c:765-974 (left), 817-1080 (right; note its own operand orders).
@@ -694,6 +741,33 @@ struct Sample {
     int rc;
 };
 
+// exact.c:332-469 after the pressure iteration: u*, then the sample on the
+// side of the contact that x/t falls on (or NaN and the failure's kind).
+HFV_HD __forceinline__ Sample exact_sample_star(double gamma, Side L, double vl, Side R, double vr, PStar P,
+                                                double speed)
+{
+    Sample S;
+    S.us = __builtin_nan("");
+    S.rc = ERR_NONE;
+    if (P.rc != ERR_NONE) {
+        const double nan = __builtin_nan("");
+        S.w = Prim{nan, nan, nan, nan};
+        S.rc = P.rc;
+        return S;
+    }
+    const double ps = P.p;
+    const double us = 0.5 * (L.u + R.u) + 0.5 * (wave_f(gamma, R.r, R.p, R.a, ps) - wave_f(gamma, L.r, L.p, L.a, ps));
+    S.us = us;
+    if (speed < us) {
+        S.w = sample_left(gamma, L, us, ps, speed);
+        S.w.v = vl;
+    } else {
+        S.w = sample_right(gamma, R, us, ps, speed);
+        S.w.v = vr;
+    }
+    return S;
+}
+
 // SHORTCUT: a problem with equal left and right states (rho, u, p) whose
 // solve runs clear of every special case (no vacuum, guess above tol, no
 // overflow) takes the solve's result directly: the PVRS guess is p, the
@@ -738,24 +812,73 @@ HFV_HD __forceinline__ Sample exact_sample(double gamma, double tol, double rl, 
         return S;
     }
     const Side L{rl, ul, pl, al}, R{rr, ur, pr, ar};
-    const PStar P = solve_p_star(gamma, L, R, tol);
-    if (P.rc != ERR_NONE) {
-        const double nan = __builtin_nan("");
-        S.w = Prim{nan, nan, nan, nan};
-        S.rc = P.rc;
+    return exact_sample_star(gamma, L, vl, R, vr, solve_p_star(gamma, L, R, tol), speed);
+}
+
+// The exact solve of a face in two parts, for a solver that sets aside the
+// faces still iterating after newton_cap Newton iterations: exact_sample's
+// answer, or rc = ERR_PAUSED with the iteration's state in *st; then
+// exact_sample_resume finishes such a face.  Together they perform the
+// operations of exact_sample (its shortcut included) on the same values.
+constexpr int ERR_PAUSED = -2;
+
+HFV_HD __forceinline__ Sample exact_sample_capped(double gamma, double tol, double rl, double ul, double vl,
+                                                  double pl, double rr, double ur, double vr, double pr,
+                                                  double speed, int newton_cap, NewtonState* st)
+{
+    const Sample D = exact_sample<true, true>(gamma, tol, rl, ul, vl, pl, rr, ur, vr, pr, speed);
+    if (D.rc != ERR_PENDING) return D;
+    const double al = (rl > 0.0) ? sound_speed(gamma, rl, pl) : 0.0;
+    const double ar = (rr > 0.0) ? sound_speed(gamma, rr, pr) : 0.0;
+    if (rl <= 0.0 || rr <= 0.0 || ((al + ar) * 2.0 / (gamma + 1.0)) <= (ur - ul)) {
+        Sample S;
+        S.us = __builtin_nan("");
+        S.rc = ERR_NONE;
+        S.w = vacuum(gamma, rl, ul, vl, pl, al, rr, ur, vr, pr, ar, speed);
         return S;
     }
-    const double ps = P.p;
-    const double us = 0.5 * (ul + ur) + 0.5 * (wave_f(gamma, rr, pr, ar, ps) - wave_f(gamma, rl, pl, al, ps));
-    S.us = us;
-    if (speed < us) {
-        S.w = sample_left(gamma, L, us, ps, speed);
-        S.w.v = vl;
-    } else {
-        S.w = sample_right(gamma, R, us, ps, speed);
-        S.w.v = vr;
+    const Side L{rl, ul, pl, al}, R{rr, ur, pr, ar};
+    *st = newton_begin(gamma, L, R, tol);
+    double ps;
+    const int r = newton_run(gamma, L, R, tol, *st, newton_cap, &ps);
+    if (r == 0) {
+        Sample S;
+        S.rc = ERR_PAUSED;
+        return S;
     }
-    return S;
+    return exact_sample_star(gamma, L, vl, R, vr, (r > 0) ? PStar{ps, ERR_NONE} : newton_finish(gamma, L, R, tol, *st),
+                             speed);
+}
+
+HFV_HD __forceinline__ Sample exact_sample_resume(double gamma, double tol, double rl, double ul, double vl,
+                                                  double pl, double rr, double ur, double vr, double pr,
+                                                  double speed, NewtonState st)
+{
+    const double al = (rl > 0.0) ? sound_speed(gamma, rl, pl) : 0.0;
+    const double ar = (rr > 0.0) ? sound_speed(gamma, rr, pr) : 0.0;
+    const Side L{rl, ul, pl, al}, R{rr, ur, pr, ar};
+    double ps;
+    const int r = newton_run(gamma, L, R, tol, st, 100, &ps);
+    return exact_sample_star(gamma, L, vl, R, vr, (r > 0) ? PStar{ps, ERR_NONE} : newton_finish(gamma, L, R, tol, st),
+                             speed);
+}
+
+// The flux of the state sampled at x/t = 0 (exact.c:471-510; NF = 4: the
+// transverse velocity's terms added).  Returns the sample's failure kind.
+template <int NF>
+HFV_HD __forceinline__ int flux_from_sample(double gamma, const Sample& S, double* f)
+{
+    const double r = S.w.r, u = S.w.u, p = S.w.p;
+    f[0] = r * u;
+    f[1] = r * u * u + p;
+    if constexpr (NF == 4) {
+        const double v = S.w.v;
+        f[2] = f[0] * v;
+        f[3] = (p * (gamma / (gamma - 1.0)) + (0.5 * r * u * u + 0.5 * r * v * v)) * u;
+    } else {
+        f[2] = (p * (gamma / (gamma - 1.0)) + 0.5 * r * u * u) * u;
+    }
+    return S.rc;
 }
 
 // Exact Godunov flux at x/t = 0.  NF = 3: exact.c:471-510.  NF = 4: the
@@ -772,17 +895,7 @@ HFV_HD __forceinline__ int flux_exact(double gamma, double tol, const double* wl
     const Sample S = exact_sample<true, DIRECT_ONLY>(gamma, tol, wl[0], wl[1], vl, wl[NF - 1], wr[0], wr[1], vr,
                                                      wr[NF - 1], 0.0);
     if (DIRECT_ONLY && S.rc == ERR_PENDING) return ERR_PENDING;
-    const double r = S.w.r, u = S.w.u, p = S.w.p;
-    f[0] = r * u;
-    f[1] = r * u * u + p;
-    if constexpr (NF == 4) {
-        const double v = S.w.v;
-        f[2] = f[0] * v;
-        f[3] = (p * (gamma / (gamma - 1.0)) + (0.5 * r * u * u + 0.5 * r * v * v)) * u;
-    } else {
-        f[2] = (p * (gamma / (gamma - 1.0)) + 0.5 * r * u * u) * u;
-    }
-    return S.rc;
+    return flux_from_sample<NF>(gamma, S, f);
 }
 
 template <int SOLVER, int NF>

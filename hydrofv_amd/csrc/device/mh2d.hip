@@ -60,40 +60,103 @@ __global__ void k_defer_reset(Deferred D)
     if (threadIdx.x == 0) {
         D.count[0] = 0ull;
         D.count[1] = 0ull;
+        D.count[2] = 0ull;
     }
 }
 
-// One queued face per lane: the full exact solve (riemann_solver_exact.c
-// via flux_exact), its flux stored at the face; a failure is reported with
-// the cell the strip pass would name.  The queue length was written by the
-// strip pass, which ran before this launch on the same stream.
+// The flux of queued face i from its sample, stored at the face; a failure
+// is reported with the cell the strip pass would name.
+__device__ __forceinline__ void finish_face(const Grid2D& G, const Deferred& D, long long i, const Sample& S,
+                                            Ctrl* out)
+{
+    double o[4];
+    const int e = flux_from_sample<4>(G.gamma, S, o);
+    const long long id = D.qid[i];
+    const long long k = id >> 1;
+    if (id & 1) {
+        // y face: (mass, my, mx, E) back to (mass, mx, my, E)
+        D.fy[0][k] = o[0];
+        D.fy[1][k] = o[2];
+        D.fy[2][k] = o[1];
+        D.fy[3][k] = o[3];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) D.fx[q][k] = o[q];
+    }
+    if (e != ERR_NONE) {
+        const long long row = k / G.pitch, c = k % G.pitch;
+        report_error(out, e, row * G.tx + c - ((id & 1) ? 0 : 1));
+    }
+}
+
+// Newton iterations a queued face may take in k_solve_faces.  Most faces
+// converge within a few; the reference's iteration can also run its 100
+// iterations and bisect (about 1 face in 100 of a blast), and one such face
+// would hold its whole wavefront.  Those faces are set aside with their
+// iteration state and finished together by k_solve_slow.
+constexpr int NEWTON_CAP = 16;
+
+// One queued face per lane: the exact solve (riemann_solver_exact.c) of
+// exact_sample_capped.  The queue length was written by the strip pass,
+// which ran before this launch on the same stream.
 __global__ void __launch_bounds__(LANES) k_solve_faces(Grid2D G, Deferred D, Ctrl* ctrl, int slot)
 {
     const long long n = (long long)min(D.count[0], (unsigned long long)D.face_cap);
-    for (long long i = (long long)blockIdx.x * LANES + threadIdx.x; i < n; i += (long long)gridDim.x * LANES) {
-        double l[4], r[4], o[4];
+    Ctrl* out = ctrl + (slot + 1) % 3;
+    const int lane = threadIdx.x;
+    for (long long i = (long long)blockIdx.x * LANES + lane; i < n; i += (long long)gridDim.x * LANES) {
+        double l[4], r[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             l[q] = D.qw[q][i];
             r[q] = D.qw[4 + q][i];
         }
-        const int e = flux_exact<4>(G.gamma, G.tol, l, r, o);
-        const long long id = D.qid[i];
-        const long long k = id >> 1;
-        if (id & 1) {
-            // y face: (mass, my, mx, E) back to (mass, mx, my, E)
-            D.fy[0][k] = o[0];
-            D.fy[1][k] = o[2];
-            D.fy[2][k] = o[1];
-            D.fy[3][k] = o[3];
-        } else {
+        NewtonState st;
+        Sample S = exact_sample_capped(G.gamma, G.tol, l[0], l[1], l[2], l[3], r[0], r[1], r[2], r[3], 0.0,
+                                       NEWTON_CAP, &st);
+        const bool paused = S.rc == ERR_PAUSED;
+        const long long j = wave_append(&D.count[2], paused, lane);
+        if (paused && j < D.slow_cap) {
+            D.sidx[j] = i;
+            D.sst[0][j] = st.p0;
+            D.sst[1][j] = st.p_hi;
+            D.sst[2][j] = st.f_hi;
+            D.sst[3][j] = st.p_lo;
+            D.sst[4][j] = st.f_lo;
+            D.sit[j] = st.it;
+            D.sbr[j] = st.bracketed;
+            continue;
+        }
+        if (paused)  // no room to set it aside: finish here
+            S = exact_sample_resume(G.gamma, G.tol, l[0], l[1], l[2], l[3], r[0], r[1], r[2], r[3], 0.0, st);
+        finish_face(G, D, i, S, out);
+    }
+}
+
+// The set-aside faces, resumed from their saved iteration state.
+__global__ void __launch_bounds__(LANES) k_solve_slow(Grid2D G, Deferred D, Ctrl* ctrl, int slot)
+{
+    const long long n = (long long)min(D.count[2], (unsigned long long)D.slow_cap);
+    Ctrl* out = ctrl + (slot + 1) % 3;
+    for (long long j = (long long)blockIdx.x * LANES + threadIdx.x; j < n; j += (long long)gridDim.x * LANES) {
+        const long long i = D.sidx[j];
+        double l[4], r[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) D.fx[q][k] = o[q];
+        for (int q = 0; q < 4; q++) {
+            l[q] = D.qw[q][i];
+            r[q] = D.qw[4 + q][i];
         }
-        if (e != ERR_NONE) {
-            const long long row = k / G.pitch, c = k % G.pitch;
-            report_error(ctrl + (slot + 1) % 3, e, row * G.tx + c - ((id & 1) ? 0 : 1));
-        }
+        NewtonState st;
+        st.p0 = D.sst[0][j];
+        st.p_hi = D.sst[1][j];
+        st.f_hi = D.sst[2][j];
+        st.df0 = 0.0;  // used by iteration 0 only
+        st.p_lo = D.sst[3][j];
+        st.f_lo = D.sst[4][j];
+        st.it = D.sit[j];
+        st.bracketed = D.sbr[j];
+        const Sample S = exact_sample_resume(G.gamma, G.tol, l[0], l[1], l[2], l[3], r[0], r[1], r[2], r[3], 0.0, st);
+        finish_face(G, D, i, S, out);
     }
 }
 
@@ -414,6 +477,8 @@ void launch_exact(hfv_mh2d* h, bool wmode, hipStream_t s)
                            h->ctrl, slot, h->D);
     }
     hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), dim3(LANES), 0, s, G, h->D, h->ctrl, slot);
+    const unsigned slow_blocks = (unsigned)std::min<long long>(1024, (h->D.slow_cap + LANES - 1) / LANES);
+    hipLaunchKernelGGL(k_solve_slow, dim3(slow_blocks), dim3(LANES), 0, s, G, h->D, h->ctrl, slot);
     hipLaunchKernelGGL(k_update_deferred, dim3(update_blocks), dim3(UPD_THREADS), 0, s, G, h->rp, h->U[0], h->U[1],
                        h->D, h->ctrl, slot);
     h->launches++;
@@ -563,7 +628,10 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
         Deferred& D = h->D;
         D.cell_cap = (long long)G.tx * G.ty;
         D.face_cap = 2 * D.cell_cap;
-        const size_t dbytes = (field * 8 + (size_t)D.face_cap * 9 + (size_t)D.cell_cap + 2) * sizeof(double);
+        // a face that cannot be set aside is finished in the first pass
+        D.slow_cap = std::max<long long>(D.face_cap / 8, 64);
+        const size_t dbytes =
+            (field * 8 + (size_t)D.face_cap * 9 + (size_t)D.cell_cap + (size_t)D.slow_cap * 7 + 4) * sizeof(double);
         if (hipMalloc(&h->defer_slab, dbytes) != hipSuccess) {
             (void)hipFree(h->slab);
             delete h;
@@ -581,6 +649,13 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
         p += D.face_cap;
         D.cells = reinterpret_cast<long long*>(p);
         p += D.cell_cap;
+        D.sidx = reinterpret_cast<long long*>(p);
+        p += D.slow_cap;
+        for (int q = 0; q < 5; q++) D.sst[q] = p + (size_t)D.slow_cap * q;
+        p += (size_t)D.slow_cap * 5;
+        D.sit = reinterpret_cast<int*>(p);
+        D.sbr = D.sit + D.slow_cap;
+        p += D.slow_cap;  // two int arrays in one double-sized stride
         D.count = reinterpret_cast<unsigned long long*>(p);
     }
     h->launches = 0;

@@ -208,8 +208,15 @@ struct Deferred {
     double* qw[8];              // queued faces: left then right state, in the solver's order
     long long* qid;             // queued faces: (q * pitch + c) * 2 + (1: face above, 0: face left)
     long long* cells;           // deferred cells: (q * pitch + c) * 8 + DeferKind
-    unsigned long long* count;  // [0] queued faces, [1] deferred cells
-    long long face_cap, cell_cap;
+    // queued faces still iterating after k_solve_faces' Newton cap: the
+    // face's queue index and its NewtonState (p0, p_hi, f_hi, p_lo, f_lo;
+    // it; bracketed), finished by k_solve_slow
+    long long* sidx;
+    double* sst[5];
+    int* sit;
+    int* sbr;
+    unsigned long long* count;  // [0] queued faces, [1] deferred cells, [2] set-aside faces
+    long long face_cap, cell_cap, slow_cap;
 };
 
 enum DeferKind : int { DEF_INTERIOR = 0, DEF_GHOST_XLO = 1, DEF_GHOST_XHI = 2, DEF_GHOST_YLO = 3, DEF_GHOST_YHI = 4 };

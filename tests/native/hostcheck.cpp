@@ -45,6 +45,27 @@ void hc_exact_sample(int shortcut, double gamma, double tol, long long n, const 
     }
 }
 
+// exact_sample_capped, then exact_sample_resume for the faces it set
+// aside: the device's two-pass face solve, face by face
+void hc_exact_sample_capped(int cap, double gamma, double tol, long long n, const double* wl, const double* wr,
+                            const double* speed, double* out, double* us, int* rc, int* paused)
+{
+    for (long long i = 0; i < n; i++) {
+        const double* l = wl + 4 * i;
+        const double* r = wr + 4 * i;
+        NewtonState st;
+        Sample S = exact_sample_capped(gamma, tol, l[0], l[1], l[2], l[3], r[0], r[1], r[2], r[3], speed[i], cap, &st);
+        paused[i] = S.rc == ERR_PAUSED;
+        if (paused[i]) S = exact_sample_resume(gamma, tol, l[0], l[1], l[2], l[3], r[0], r[1], r[2], r[3], speed[i], st);
+        out[4 * i + 0] = S.w.r;
+        out[4 * i + 1] = S.w.u;
+        out[4 * i + 2] = S.w.v;
+        out[4 * i + 3] = S.w.p;
+        us[i] = S.us;
+        rc[i] = S.rc;
+    }
+}
+
 // sizes and selected field offsets of the structs of hfv_gpu.h, for the
 // ctypes mirrors: out[0..3] sizeof(mh2d, mh1d, run_status, adj2d config),
 // out[4..] offsets of the fields added last to each config

@@ -395,3 +395,45 @@ def test_exact_solver_equal_state_shortcut_is_bitwise(hc):
             assert np.array_equal(outs[0][0], outs[1][0]), gamma
             assert np.array_equal(outs[0][1], outs[1][1]), gamma
             assert np.array_equal(outs[0][2], outs[1][2]), gamma
+
+
+def test_exact_solver_capped_and_resumed_is_bitwise(hc):
+    """The device's two-pass face solve (exact_sample_capped: at most `cap`
+    Newton iterations, the rest set aside; exact_sample_resume: the set-aside
+    faces from their saved state) returns the bits of one uninterrupted
+    exact_sample, failure kinds included, for faces that converge early,
+    late, take the bisection path or fail, and for any cap."""
+    hc.hc_exact_sample.argtypes = [C.c_int, C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp, _dp, _dp, _ip]
+    hc.hc_exact_sample_capped.argtypes = [C.c_int, C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp, _dp, _dp,
+                                          _ip, _ip]
+    r = np.random.default_rng(13)
+    n = 20_000
+    parts = []
+    # ordinary states, blast-like pressure ratios (the Newton loop of the
+    # reference often runs out there), near-vacuum rarefactions, and NaN
+    rho = r.uniform(0.1, 2, (n, 2)); p = r.uniform(0.1, 2, (n, 2)); u = r.uniform(-1, 1, (n, 2))
+    parts.append((rho, u, p))
+    rho = 10 ** r.uniform(-2, 1, (n, 2)); p = 10 ** r.uniform(-6, 5, (n, 2)); u = r.uniform(-30, 30, (n, 2))
+    parts.append((rho, u, p))
+    rho = 10 ** r.uniform(-4, 0, (n, 2)); p = 10 ** r.uniform(-8, 0, (n, 2)); u = r.uniform(-5, 5, (n, 2))
+    u[:, 1] = u[:, 0] + r.uniform(0, 8, n)
+    parts.append((rho, u, p))
+    rho, u, p = (np.concatenate(x) for x in zip(*parts))
+    p[:50, 0] = np.nan
+    m = rho.shape[0]
+    v = r.normal(size=(m, 2))
+    wl = np.ascontiguousarray(np.stack([rho[:, 0], u[:, 0], v[:, 0], p[:, 0]], axis=1))
+    wr = np.ascontiguousarray(np.stack([rho[:, 1], u[:, 1], v[:, 1], p[:, 1]], axis=1))
+    speed = np.where(r.uniform(size=m) < 0.5, 0.0, r.uniform(-20, 20, m))
+    for tol in (1e-6, 1e-8):
+        out0, us0, rc0 = np.empty((m, 4)), np.empty(m), np.empty(m, dtype=np.int32)
+        hc.hc_exact_sample(1, 1.4, tol, m, wl, wr, speed, out0, us0, rc0)
+        for cap in (1, 2, 16, 99):
+            out, us, rc, paused = np.empty((m, 4)), np.empty(m), np.empty(m, dtype=np.int32), np.empty(m, dtype=np.int32)
+            hc.hc_exact_sample_capped(cap, 1.4, tol, m, wl, wr, speed, out, us, rc, paused)
+            assert np.array_equal(out.view(np.int64), out0.view(np.int64)), (tol, cap)
+            assert np.array_equal(us.view(np.int64), us0.view(np.int64)), (tol, cap)
+            assert np.array_equal(rc, rc0), (tol, cap)
+            if cap == 16:
+                assert 0 < paused.sum() < m
+        assert (rc0 != 0).sum() >= 50  # the NaN faces fail (bisection)
