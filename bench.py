@@ -83,11 +83,8 @@ def compute_bound(ms_per_step):
 def sod_run(lib, n=1000):
     """Benchmark configuration 'Sod 1D, N=1000, HLLC + minmod, CFL 0.9, to
     t=0.2' as one muscl_hancock_1d call (the whole run in one device
-    workgroup), against the reference build's serial CPU run of the same."""
-    import ctypes as C
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import _oracle as O
-    from hydrofv_amd import abi, problems
+    workgroup)."""
+    from hydrofv_amd import problems
     out = {"cells": n, "tf": 0.2}
     sim = problems.sod(lib, n)
     sim.launch("muscl_hancock_1d", "riemann_solver_hllc", "minmod", 0.9, 0.2)  # warm-up
@@ -98,24 +95,34 @@ def sod_run(lib, n=1000):
     out["device_ms"] = (time.perf_counter() - t0) * 1e3
     out["steps"] = res.num_steps
     sim.close()
-    if os.path.exists(O.REF_SO):
-        ref = abi.bind(C.CDLL(O.REF_SO))
-        sim = problems.sod(ref, n)
-        t0 = time.perf_counter()
-        rres = sim.launch("muscl_hancock_1d", "riemann_solver_hllc", "minmod", 0.9, 0.2)
-        out["cpu_reference_serial_ms"] = (time.perf_counter() - t0) * 1e3
-        out["cpu_steps"] = rres.num_steps
-        sim.close()
     out["note"] = "wall time of the library call, host-device copies included"
     return out
 
 
-def sedov_run(lib, n=2048, tf=0.05):
+def cpu_baseline_sod(n=1000):
+    """The reference's serial build (oracle/_ref) on the same Sod run, beside
+    the device's: part of the CPU-baseline leg."""
+    import ctypes as C
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    from hydrofv_amd import abi, problems
+    if not os.path.exists(O.REF_SO):
+        return None
+    ref = abi.bind(C.CDLL(O.REF_SO))
+    sim = problems.sod(ref, n)
+    t0 = time.perf_counter()
+    res = sim.launch("muscl_hancock_1d", "riemann_solver_hllc", "minmod", 0.9, 0.2)
+    ms = (time.perf_counter() - t0) * 1e3
+    sim.close()
+    return {"cpu_reference_serial_ms": ms, "cpu_steps": res.num_steps, "cores": 1, "kind": "reference"}
+
+
+def sedov_run(lib, n=2048, tf=0.05, use_graph=True):
     """Benchmark configuration 'Sedov 2D 2048^2, exact Riemann solver (tol
     1e-8) + Van Leer, CFL 0.5, run to t = 0.05': the whole run, state
     resident on the device, steps queued as captured graphs between status
-    checks.  Reports the wall time of the run (first step to the end) and
-    its mean cell-updates/s."""
+    checks (use_graph False: plain launches).  Reports the wall time of the
+    run (first step to the end) and its mean cell-updates/s."""
     import torch
     from hydrofv_amd import gpu, problems
     sim = problems.sedov(lib, n)
@@ -129,7 +136,7 @@ def sedov_run(lib, n=2048, tf=0.05):
     t0 = time.perf_counter()
     dev.start(stream=sh)
     while True:
-        dev.advance(960, use_graph=True, stream=sh)
+        dev.advance(960, use_graph=use_graph, stream=sh)
         st = dev.status(stream=sh)
         if st.finished:
             break
@@ -188,7 +195,8 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="plain launches instead of captured graphs (the KH and Sedov legs)")
     ap.add_argument("--adjoint-solves", type=int, default=32,
                     help="gradient solves of the 1024^2, 500-step adjoint workload (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=96,
@@ -297,12 +305,18 @@ def main():
 
     extra = {}
     if not args.no_extra_configs:
-        for name, fn in (("sod_1d", sod_run), ("sedov_2d", sedov_run)):
+        legs = (("sod_1d", lambda: sod_run(lib)), ("sedov_2d", lambda: sedov_run(lib, use_graph=not args.no_graph)))
+        for name, fn in legs:
             try:
-                extra[name] = fn(lib)
+                extra[name] = fn()
                 log(f"{name}: {extra[name]}")
             except Exception as e:  # the main figure stands without them
                 log(f"{name} failed: {e!r}")
+        if "sod_1d" in extra and not args.no_cpu_baseline:
+            try:
+                extra["sod_1d"]["cpu_baseline"] = cpu_baseline_sod()
+            except Exception as e:
+                log(f"sod cpu baseline failed: {e!r}")
 
     traffic = measured_traffic()
     line = {
