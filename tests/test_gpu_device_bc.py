@@ -51,6 +51,23 @@ def test_device_hooks_match_oracle(gpulib, name, mk, integrator, limiter, steps)
     assert run_and_compare(gpulib, name, mk, integrator, limiter, steps) == 0
 
 
+def test_device_hooks_with_the_exact_solver(gpulib):
+    """The exact solver's queued-face step under device hooks: every step
+    reads the primitive state the device fill has just refreshed."""
+    calls = []
+    sim = K.setup_all_custom(gpulib, 40, 23, 2, 9, device=True, calls=calls)
+    grid = K.oracle_grid(sim, "van_leer", "riemann_solver_exact")
+    before = len(calls)
+    res = sim.launch("muscl_hancock_2d", "riemann_solver_exact", "van_leer", 0.5, np.inf, max_steps=12)
+    assert len(calls) == before
+    r = grid.run(0.5, max_steps=12)
+    assert r.error == 0 and res.num_steps == r.num_steps == 12
+    assert abs(res.t - r.t) <= 1e-10 * r.t
+    W, U = K.state(sim)
+    compare("device hooks exact", W, U, grid.W.reshape(W.shape), grid.U.reshape(U.shape), tol=1e-10)
+    sim.close()
+
+
 def test_partly_registered_hooks_keep_the_host_path(gpulib):
     """Two of four custom sides with a device form: the run takes the host
     path, one host call per custom side and step."""
