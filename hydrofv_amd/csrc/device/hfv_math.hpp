@@ -783,11 +783,20 @@ template <bool SHORTCUT = true, bool DIRECT_ONLY = false>
 HFV_HD __forceinline__ Sample exact_sample(double gamma, double tol, double rl, double ul, double vl, double pl,
                                            double rr, double ur, double vr, double pr, double speed)
 {
-    const double al = (rl > 0.0) ? sound_speed(gamma, rl, pl) : 0.0;
-    const double ar = (rr > 0.0) ? sound_speed(gamma, rr, pr) : 0.0;
     Sample S;
     S.us = __builtin_nan("");
     S.rc = ERR_NONE;
+    if constexpr (DIRECT_ONLY) {
+        // only the shortcut can answer: test the states first; when they are
+        // equal with rl > 0 and pl >= tol > 0 they have the same bits, so
+        // the right sound speed is the left one
+        if (!((rl == rr) & (ul == ur) & (pl == pr))) {
+            S.rc = ERR_PENDING;
+            return S;
+        }
+    }
+    const double al = (rl > 0.0) ? sound_speed(gamma, rl, pl) : 0.0;
+    const double ar = DIRECT_ONLY ? al : ((rr > 0.0) ? sound_speed(gamma, rr, pr) : 0.0);
     if constexpr (SHORTCUT) {
         const bool same = (rl == rr) & (ul == ur) & (pl == pr) & (rl > 0.0) & (pl >= tol) & (al > 0.0);
         const bool finite = (fabs(rl + rr) < INFINITY) & (fabs(pl + pr) < INFINITY) & (fabs(al + ar) < INFINITY) &

@@ -424,6 +424,11 @@ def test_exact_solver_capped_and_resumed_is_bitwise(hc):
     v = r.normal(size=(m, 2))
     wl = np.ascontiguousarray(np.stack([rho[:, 0], u[:, 0], v[:, 0], p[:, 0]], axis=1))
     wr = np.ascontiguousarray(np.stack([rho[:, 1], u[:, 1], v[:, 1], p[:, 1]], axis=1))
+    # a tenth with equal states (the shortcut, answered before any iteration)
+    eq = r.uniform(size=m) < 0.1
+    wr[eq, 0] = wl[eq, 0]
+    wr[eq, 1] = wl[eq, 1]
+    wr[eq, 3] = wl[eq, 3]
     speed = np.where(r.uniform(size=m) < 0.5, 0.0, r.uniform(-20, 20, m))
     for tol in (1e-6, 1e-8):
         out0, us0, rc0 = np.empty((m, 4)), np.empty(m), np.empty(m, dtype=np.int32)
