@@ -99,15 +99,17 @@ __device__ __forceinline__ void load_prim(const Grid2D& G, const Fields4& U, con
 }
 
 // y-predictor of the cell whose own state is wc, with wb below and wt above.
+// Returns the cell's sound speed, which its x-predictor uses a row later.
 template <int LIM>
-__device__ __forceinline__ void ypredict(const double* wb, const double* wc, const double* wt, double c, double gamma,
-                                         double* lo, double* hi)
+__device__ __forceinline__ double ypredict(const double* wb, const double* wc, const double* wt, double c,
+                                           double gamma, double* lo, double* hi)
 {
     double d[4];
 #pragma unroll
     for (int f = 0; f < 4; f++) d[f] = limit<LIM>(wc[f] - wb[f], wt[f] - wc[f]);
     const double a = sound_speed(gamma, wc[0], wc[3]);
     predict_y(wc, d, c, a, lo, hi);
+    return a;
 }
 
 // y-sweep flux between the top face of the lower cell (hi) and the bottom
@@ -332,6 +334,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
 
     // ---- prologue: rows y0-2 .. y0+1 (first order: y0-1 .. y0) ----------
     double wa[4], wb[4], wc[4];
+    double a_row = 0.0, a_next = 0.0;  // sound speeds of rows r and r+1 (MUSCL)
     double yt[4], fyb[4];
     if constexpr (LIM == FIRST_ORDER) {
         double wm1[4];
@@ -356,8 +359,8 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         load_prim<WMODE>(G, U, W, y0, col, wa);
         load_prim<WMODE>(G, U, W, y0 + 1, col, wb);
         double lo[4], hi_m1[4], lo0[4];
-        ypredict<LIM>(wm2, wm1, wa, cy, gamma, lo, hi_m1);   // row y0-1: top face state
-        ypredict<LIM>(wm1, wa, wb, cy, gamma, lo0, yt);      // row y0: bottom and top face states
+        ypredict<LIM>(wm2, wm1, wa, cy, gamma, lo, hi_m1);         // row y0-1: top face state
+        a_row = ypredict<LIM>(wm1, wa, wb, cy, gamma, lo0, yt);    // row y0: bottom and top face states
         if constexpr (DEFER) {
             hyb = out_col && direct_face<true>(gamma, tol, hi_m1, lo0, fyb);
             queue_face<true>(D, hyb && y0 == g, (long long)(y0 - 1) * G.pitch + col, hi_m1, lo0, lane);
@@ -393,7 +396,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             }
         } else {
             load_prim<WMODE>(G, U, W, r + 2, col, wc);
-            ypredict<LIM>(wa, wb, wc, cy, gamma, ylo, yhi);
+            a_next = ypredict<LIM>(wa, wb, wc, cy, gamma, ylo, yhi);
             double wl[4], wr[4];
 #pragma unroll
             for (int f = 0; f < 4; f++) {
@@ -403,8 +406,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             double d[4];
 #pragma unroll
             for (int f = 0; f < 4; f++) d[f] = limit<LIM>(wa[f] - wl[f], wr[f] - wa[f]);
-            const double a = sound_speed(gamma, wa[0], wa[3]);
-            predict_x(wa, d, cx, a, xlo, xhi);
+            predict_x(wa, d, cx, a_row, xlo, xhi);
         }
         double hl[4];
 #pragma unroll
@@ -544,6 +546,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             yt[f] = yhi[f];
             fyb[f] = fyt[f];
         }
+        a_row = a_next;
         hyb = hy;
     }
     if constexpr (DEFER) {
