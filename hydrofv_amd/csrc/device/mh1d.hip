@@ -243,54 +243,75 @@ __global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays
                 break;
             }
         } else {
-        // reconstruct cells g-1 .. g+n (146-246)
-        for (int i = t; i < n + 2; i += T1D) {
-            const int k = g + i - 1;
+        // interface fluxes 0..n (252-295), each from the reconstructions
+        // (146-246) of its two cells g+i-1 and g+i, which the lane computes
+        // itself: no reconstruction arrays and no barrier between the two.
+        // A failed flux is not reported: the reference overwrites the flux
+        // loop's error status with that of the boundary fill that follows
+        // (296-321), so the run continues.
+        auto reconstruct = [&](int k, double* lo, double* hi) {
             const double s[3] = {A.w[0][k], A.w[1][k], A.w[2][k]};
             const double cc = dt * A.area[k] / A.vol[k];
             const double a = sound_speed(gamma, s[0], s[2]);
             double d[3];
             for (int f = 0; f < 3; f++) d[f] = limit<LIM>(s[f] - A.w[f][k - 1], A.w[f][k + 1] - s[f]);
-            double lo[3], hi[3];
             predict_1d(s, d, cc, a, lo, hi);
-            for (int f = 0; f < 3; f++) {
-                A.lo[f][i] = lo[f];
-                A.hi[f][i] = hi[f];
-            }
-        }
-        __syncthreads();
-        // interface fluxes 0..n (252-295).  A failed flux is not reported:
-        // the reference overwrites the flux loop's error status with that of
-        // the boundary fill that follows (296-321), so the run continues.
+        };
         for (int i = t; i <= n; i += T1D) {
-            const double l[3] = {A.hi[0][i], A.hi[1][i], A.hi[2][i]};
-            const double r[3] = {A.lo[0][i + 1], A.lo[1][i + 1], A.lo[2][i + 1]};
+            double lo_l[3], hi_l[3], lo_r[3], hi_r[3];
+            reconstruct(g + i - 1, lo_l, hi_l);
+            reconstruct(g + i, lo_r, hi_r);
             double f[3];
-            if (flux<SOLVER, 3>(gamma, G.tol, l, r, f) != ERR_NONE) {
+            if (flux<SOLVER, 3>(gamma, G.tol, hi_l, lo_r, f) != ERR_NONE) {
                 f[0] = f[1] = f[2] = __builtin_nan("");
             }
             for (int q = 0; q < 3; q++) A.fl[q][i] = f[q] * dt;
         }
         }
+        bool key_done = false;
         if constexpr (LIM != RANDOM_CHOICE) {
             __syncthreads();
-            // update in the reference's order: a cell first gains its left
-            // face's flux, then loses its right face's, both scaled by its own
-            // face-array entry (287-294)
-            for (int k = g - 1 + t; k <= g + n; k += T1D) {
-                const double ak = A.area[k];
-                for (int q = 0; q < 3; q++) {
-                    double x = A.u[q][k];
-                    if (k >= g) x += A.fl[q][k - g] * ak;
-                    if (k < g + n) x -= A.fl[q][k - g + 1] * ak;
-                    A.u[q][k] = x;
+            // per cell, by one lane: the update in the reference's order (a
+            // cell first gains its left face's flux, then loses its right
+            // face's, both scaled by its own face-array entry, 287-294), then
+            // c2p (297-321), then, without a source term, its share of the
+            // CFL maximum of the interior (utils.c:28-53; the boundary fill
+            // leaves the interior as it is)
+            const bool fused_cfl = !(G.coord == 4 || G.coord == 5);
+            double smax = -INFINITY;
+            for (int k = t; k < G.tx; k += T1D) {
+                if (k >= g - 1 && k <= g + n) {
+                    const double ak = A.area[k];
+                    for (int q = 0; q < 3; q++) {
+                        double x = A.u[q][k];
+                        if (k >= g) x += A.fl[q][k - g] * ak;
+                        if (k < g + n) x -= A.fl[q][k - g + 1] * ak;
+                        A.u[q][k] = x;
+                    }
+                }
+                double w[3];
+                c2p_1d(G.gamma, A.vol[k], A.u[0][k], A.u[1][k], A.u[2][k], w);
+                A.w[0][k] = w[0];
+                A.w[1][k] = w[1];
+                A.w[2][k] = w[2];
+                if (fused_cfl && k >= g && k < g + n) {
+                    const double v = sound_speed(gamma, w[0], w[2]) + fabs(w[1]);
+                    if (v > smax) smax = v;
                 }
             }
             __syncthreads();
-            c2p_all(G, A, t);
-            __syncthreads();
             bc_1d(G, A.w, t);
+            if (fused_cfl) {
+                smax = wave_max(smax);
+                if ((t & 63) == 0) red[t >> 6] = smax;
+            }
             __syncthreads();
+            if (fused_cfl) {
+                double m = red[0];
+                for (int k = 1; k < T1D / 64; k++) m = (red[k] > m) ? red[k] : m;
+                c.key[0] = smax_key(m);
+                key_done = true;
+            }
         }
         if (G.coord == 4 || G.coord == 5) {
             // source_term.c:45-175: RK4 per interior cell, then c2p and fill
@@ -321,7 +342,7 @@ __global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays
         c.t = plan.t + dt;
         c.dt = dt;
         c.num_steps = plan.n + 1;
-        c.key[0] = smax_key(cfl_max());
+        if (!key_done) c.key[0] = smax_key(cfl_max());
     }
     if (t == 0) *ctrl = c;
 }
