@@ -209,7 +209,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested).",
+        print(f"bench.py measures one GPU: this library has no multi-GPU path (--gpus {args.gpus} requested).",
               file=sys.stderr)
         sys.exit(2)
 

@@ -3,8 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of hydroFV's
  * MUSCL-Hancock update, used by tests/, by the smoke check and by the CPU
- * baseline leg of bench.py as the checker.  The product (hydrofv_amd) never
- * links, loads or calls it.
+ * baseline leg of bench.py to check results.  The product (hydrofv_amd)
+ * never links, loads or calls it.
  *
  * Every function follows the arithmetic order of the reference function it
  * cites so that results are bitwise equal to the reference built with

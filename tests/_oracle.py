@@ -1,6 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so) and of the reference
 build (oracle/_ref/libhydro_ref.so).  Test infrastructure: used only by tests,
-the smoke check and the CPU-baseline leg of the benchmark, as the checker."""
+the smoke check and the CPU-baseline leg of the benchmark, to check results
+and time the CPU, never as the thing measured or shipped."""
 from __future__ import annotations
 
 import ctypes as C
