@@ -155,6 +155,9 @@ HFV_HD __forceinline__ double pow_with_log(double x, double y, DD l)
     const bool unf = wh < -745.2;
     const bool inrange = !(ovf | unf) & (wh == wh);
     double r = exp_dd(DD{inrange ? wh : 0.0, inrange ? wl : 0.0});
+    // a positive finite x other than 1 and a nonzero y with x^y in range:
+    // every special case below leaves r as it is
+    if (regular & (x > 0.0) & inrange & (x != 1.0) & (y != 0.0)) return r;
     r = ovf ? INFINITY : r;
     r = unf ? 0.0 : r;
     // |x| = 0 or infinity

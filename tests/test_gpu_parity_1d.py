@@ -30,11 +30,22 @@ def run_1d(gpulib, setup, limiter, solver, cfl, tf=np.inf, steps=None, tol=1e-6,
 
 
 def test_sod_1000_hllc_minmod_bitwise(gpulib):
-    """Sod, N=1000, HLLC + minmod, CFL 0.9, to t=0.2 (497 steps)."""
+    """Sod, N=1000, HLLC + minmod, CFL 0.9, to t=0.2 (497 steps); the device
+    result also matches the exact Riemann solution: L1 errors of rho, u and p
+    (the oracle's exact sampler at tol 1e-12, cell centres) at the level of a
+    second-order scheme on 1000 cells (about 9e-4, 9e-4 and 5e-4)."""
+    import _oracle as O
     W, U, Wo, Uo, res = run_1d(gpulib, lambda lib: K.setup_sod(lib, 1000), "minmod", "riemann_solver_hllc",
                                0.9, tf=0.2)
     assert res.num_steps == 497
     compare("sod", W[:, None], U[:, None], Wo[:, None], Uo[:, None])
+    g, n = 2, 1000
+    xi = ((np.arange(n + 2 * g) + 0.5) / (n + 2 * g))[g:g + n]
+    sol = O.oracle_exact_sample(1.4, 1e-12, [1.0, 0.0, 1.0], [0.125, 0.0, 0.1], (xi - 0.5) / 0.2)
+    h = 1.0 / (n + 2 * g)
+    l1 = [float(np.sum(np.abs(W[k][g:g + n] - sol[:, k])) * h) for k in range(3)]
+    print("Sod L1 vs exact (rho, u, p):", l1)
+    assert all(e < 1.2e-3 for e in l1), l1
 
 
 @pytest.mark.parametrize("limiter", K.LIMITERS)
