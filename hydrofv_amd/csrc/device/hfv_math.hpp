@@ -316,12 +316,21 @@ struct HllcFace {
 // evaluations of a lane's faces; each selected value comes from the same
 // expression as in flux_hllc, so the bits are the same.  F[i].wl and F[i].wr
 // are set by the caller.
+// The guess of hllc_begin_faces in three phases, so that a caller can take
+// the two-shock candidates of only the faces that use them (strip_step packs
+// them across the wavefront): hllc_guess_begin (sound speeds, vacuum test,
+// PVRS value and which guess applies), tsrs_guess for each face that needs
+// it, hllc_guess_end (the choice).
 template <int N>
-HFV_HD __forceinline__ void hllc_begin_faces(double gamma, double tol, HllcFace (&F)[N])
-{
-    const double h = 0.5 * (gamma - 1.0);
+struct HllcGuess {
     double ppv[N], num[N], tsrs[N];
     bool vac[N], pvrs[N], trrs[N];
+};
+
+template <int N>
+HFV_HD __forceinline__ void hllc_guess_begin(double gamma, HllcFace (&F)[N], HllcGuess<N>& G)
+{
+    const double h = 0.5 * (gamma - 1.0);
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const double rl = F[i].wl[0], ul = F[i].wl[1], pl = F[i].wl[3];
@@ -331,31 +340,58 @@ HFV_HD __forceinline__ void hllc_begin_faces(double gamma, double tol, HllcFace 
         const double ar = (rr > 0.0) ? ar_ : 0.0;
         F[i].al = al;
         F[i].ar = ar;
-        vac[i] = hllc_is_vacuum(gamma, rl, ul, al, rr, ur, ar);
+        G.vac[i] = hllc_is_vacuum(gamma, rl, ul, al, rr, ur, ar);
         // riemann_solver.c:154-224 (as guess_select)
         const double plo = (pl > pr) ? pr : pl;
         const double phi = (pl > pr) ? pl : pr;
-        ppv[i] = 0.5 * (pl + pr) - 0.125 * (ur - ul) * (rl + rr) * (al + ar);
-        pvrs[i] = (phi / plo <= 2.0) & (plo <= ppv[i]) & (ppv[i] <= phi);
-        trrs[i] = !pvrs[i] & (ppv[i] < plo);
-        num[i] = al + ar - h * (ur - ul);
+        G.ppv[i] = 0.5 * (pl + pr) - 0.125 * (ur - ul) * (rl + rr) * (al + ar);
+        G.pvrs[i] = (phi / plo <= 2.0) & (plo <= G.ppv[i]) & (G.ppv[i] <= phi);
+        G.trrs[i] = !G.pvrs[i] & (G.ppv[i] < plo);
+        G.num[i] = al + ar - h * (ur - ul);
+        G.tsrs[i] = 0.0;
     }
+}
+
+// The two-shock guess (riemann_solver.c:201-215) from the face's pressures
+// and densities, du = ur - ul and the PVRS value.
+HFV_HD __forceinline__ double tsrs_guess(double gamma, double rl, double pl, double rr, double pr, double du,
+                                         double ppv)
+{
+    const double gl = sqrt(coeff_A(gamma, rl) / (ppv + coeff_B(gamma, pl)));
+    const double gr = sqrt(coeff_A(gamma, rr) / (ppv + coeff_B(gamma, pr)));
+    return (gl * pl + gr * pr - du) / (gl + gr);
+}
+
+// faces that take the two-shock guess (not vacuum, not two-rarefaction, not PVRS)
+template <int N>
+HFV_HD __forceinline__ bool needs_tsrs(const HllcGuess<N>& G, int i)
+{
+    return !G.vac[i] & !G.trrs[i] & !G.pvrs[i];
+}
+
+template <int N>
+HFV_HD __forceinline__ void hllc_guess_end(double tol, HllcFace (&F)[N], const HllcGuess<N>& G)
+{
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double g = G.pvrs[i] ? G.ppv[i] : G.tsrs[i];
+        g = (g < tol) ? tol : g;
+        F[i].ps = G.vac[i] ? 0.0 : (G.trrs[i] ? G.num[i] : g);
+        F[i].kind = G.vac[i] ? HLLC_VACUUM : (G.trrs[i] ? HLLC_TRRS : HLLC_READY);
+    }
+}
+
+template <int N>
+HFV_HD __forceinline__ void hllc_begin_faces(double gamma, double tol, HllcFace (&F)[N])
+{
+    HllcGuess<N> G;
+    hllc_guess_begin<N>(gamma, F, G);
     // the two-shock candidates as a phase of their own
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-        const double rl = F[i].wl[0], ul = F[i].wl[1], pl = F[i].wl[3];
-        const double rr = F[i].wr[0], ur = F[i].wr[1], pr = F[i].wr[3];
-        const double gl = sqrt(coeff_A(gamma, rl) / (ppv[i] + coeff_B(gamma, pl)));
-        const double gr = sqrt(coeff_A(gamma, rr) / (ppv[i] + coeff_B(gamma, pr)));
-        tsrs[i] = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
-    }
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-        double g = pvrs[i] ? ppv[i] : tsrs[i];
-        g = (g < tol) ? tol : g;
-        F[i].ps = vac[i] ? 0.0 : (trrs[i] ? num[i] : g);
-        F[i].kind = vac[i] ? HLLC_VACUUM : (trrs[i] ? HLLC_TRRS : HLLC_READY);
-    }
+    for (int i = 0; i < N; i++)
+        G.tsrs[i] = tsrs_guess(gamma, F[i].wl[0], F[i].wl[3], F[i].wr[0], F[i].wr[3], F[i].wr[1] - F[i].wl[1],
+                               G.ppv[i]);
+    hllc_guess_end<N>(tol, F, G);
 }
 
 HFV_HD __forceinline__ void hllc_begin(double gamma, double tol, const double* wl, const double* wr, HllcFace& F)

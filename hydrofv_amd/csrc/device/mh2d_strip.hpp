@@ -191,6 +191,55 @@ __device__ __forceinline__ void trrs_powers(double gamma, double tol, HllcFace& 
     __syncthreads();
 }
 
+// The two-shock guesses of the two faces per lane (riemann_solver.c:
+// 201-215), taken only for the faces that use them: their inputs are packed
+// into LDS, each lane computes one packed face per pass, and the results are
+// read back.  About 30 faces in 128 take it on a Kelvin-Helmholtz run, so one
+// pass replaces evaluating it on every face.  q: 6 * 64 doubles of LDS; the
+// workgroup is this one wavefront, and every lane must call this.
+__device__ __forceinline__ void tsrs_compact(double gamma, const HllcFace (&F)[2], HllcGuess<2>& G, double* q,
+                                             int lane)
+{
+    const bool ta = needs_tsrs<2>(G, 0), tb = needs_tsrs<2>(G, 1);
+    const unsigned long long ma = __ballot(ta), mb = __ballot(tb);
+    if ((ma | mb) == 0ull) return;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int na = __popcll(ma), n = na + __popcll(mb);
+    const int ia = __popcll(ma & below), ib = na + __popcll(mb & below);
+    for (int b = 0; b < n; b += LANES) {
+        // input j of packed face s at q[j * LANES + s]
+        const bool pa = ta && ia >= b && ia < b + LANES, pb = tb && ib >= b && ib < b + LANES;
+        if (pa) {
+            const int sl = ia - b;
+            q[0 * LANES + sl] = F[0].wl[0];
+            q[1 * LANES + sl] = F[0].wl[3];
+            q[2 * LANES + sl] = F[0].wr[0];
+            q[3 * LANES + sl] = F[0].wr[3];
+            q[4 * LANES + sl] = F[0].wr[1] - F[0].wl[1];
+            q[5 * LANES + sl] = G.ppv[0];
+        }
+        if (pb) {
+            const int sl = ib - b;
+            q[0 * LANES + sl] = F[1].wl[0];
+            q[1 * LANES + sl] = F[1].wl[3];
+            q[2 * LANES + sl] = F[1].wr[0];
+            q[3 * LANES + sl] = F[1].wr[3];
+            q[4 * LANES + sl] = F[1].wr[1] - F[1].wl[1];
+            q[5 * LANES + sl] = G.ppv[1];
+        }
+        __syncthreads();
+        if (b + lane < n) {
+            const double t = tsrs_guess(gamma, q[0 * LANES + lane], q[1 * LANES + lane], q[2 * LANES + lane],
+                                        q[3 * LANES + lane], q[4 * LANES + lane], q[5 * LANES + lane]);
+            q[lane] = t;  // this lane's own slot, its inputs already read
+        }
+        __syncthreads();
+        if (pa) G.tsrs[0] = q[ia - b];
+        if (pb) G.tsrs[1] = q[ib - b];
+        __syncthreads();
+    }
+}
+
 __device__ __forceinline__ double shfl_from_left(double v) { return __shfl_up(v, 1, LANES); }
 __device__ __forceinline__ double shfl_from_right(double v) { return __shfl_down(v, 1, LANES); }
 
@@ -434,7 +483,10 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
                 ey = hllc_end_branchy(gamma, F[0], o[0]);
                 ex = hllc_end_branchy(gamma, F[1], o[1]);
             } else {
-                hllc_begin_faces<2>(gamma, tol, F);
+                HllcGuess<2> Gs;
+                hllc_guess_begin<2>(gamma, F, Gs);
+                tsrs_compact(gamma, F, Gs, trrs_q, lane);
+                hllc_guess_end<2>(tol, F, Gs);
                 trrs_powers(gamma, tol, F[0], F[1], trrs_q, lane);
                 // both faces' regular fluxes together, then the rare vacuum faces
                 int e2[2];
