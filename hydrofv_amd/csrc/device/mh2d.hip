@@ -94,7 +94,7 @@ __device__ __forceinline__ void finish_face(const Grid2D& G, const Deferred& D, 
 // iterations and bisect (about 1 face in 100 of a blast), and one such face
 // would hold its whole wavefront.  Those faces are set aside with their
 // iteration state and finished together by k_solve_slow.
-constexpr int NEWTON_CAP = 16;
+constexpr int NEWTON_CAP = 8;
 
 // One queued face per lane: the exact solve (riemann_solver_exact.c) of
 // exact_sample_capped.  The queue length was written by the strip pass,
