@@ -122,14 +122,28 @@ __device__ __forceinline__ void geo_rate(double pre, double gamma, double rho, d
 // of the MUSCL-Hancock step.  LIM == RANDOM_CHOICE: random_choice_1d, which
 // samples W directly and then fills the boundary, converts to conserved
 // variables and adds the geometric source.
+// lds_stride > 0: the primitive and flux arrays live in LDS (6 arrays of
+// lds_stride doubles of dynamic shared memory) for the launch; W is copied in
+// at its start and back at its end.
 template <int LIM, int SOLVER>
-__global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays1D A, Ctrl* ctrl, long long budget)
+__global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays1D Ain, Ctrl* ctrl, long long budget,
+                                                  int lds_stride)
 {
+    extern __shared__ double lds[];
     __shared__ double red[T1D / 64];
     __shared__ int fail_i, fail_last, fail_kind;
     const int t = threadIdx.x;
     const int g = G.g, n = G.n;
     const double gamma = G.gamma;
+    Arrays1D A = Ain;
+    if (lds_stride > 0) {
+        for (int f = 0; f < 3; f++) {
+            A.w[f] = lds + f * lds_stride;
+            A.fl[f] = lds + (3 + f) * lds_stride;
+        }
+        for (int k = t; k < G.tx; k += T1D)
+            for (int f = 0; f < 3; f++) A.w[f][k] = Ain.w[f][k];
+    }
     if (t == 0) {
         fail_i = INT_MAX;
         fail_last = -1;
@@ -344,6 +358,11 @@ __global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays
         c.num_steps = plan.n + 1;
         if (!key_done) c.key[0] = smax_key(cfl_max());
     }
+    if (lds_stride > 0) {
+        __syncthreads();
+        for (int k = t; k < G.tx; k += T1D)
+            for (int f = 0; f < 3; f++) Ain.w[f][k] = A.w[f][k];
+    }
     if (t == 0) *ctrl = c;
 }
 
@@ -534,9 +553,15 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
     const int lim = cfg->slope_limiter;
     const bool hllc = cfg->riemann_solver == HLLC;
     const long long budget = 1LL << 16;  // steps per launch
+    // W and the fluxes in LDS when they fit in 60 KiB (about 1250 cells)
+    const int lds_stride = tx + 2;
+    const bool lds_fits = (size_t)6 * lds_stride * sizeof(double) <= 60 * 1024;
+    const unsigned lds_bytes = lds_fits ? (unsigned)(6 * lds_stride * sizeof(double)) : 0u;
+    const int lds_arg = lds_fits ? lds_stride : 0;
     Ctrl c;
     for (;;) {
-#define HFV_L1D(L, S) hipLaunchKernelGGL((k_mh1d_run<L, S>), dim3(1), dim3(T1D), 0, s, G, rp, A, dctrl.p, budget)
+#define HFV_L1D(L, S) \
+    hipLaunchKernelGGL((k_mh1d_run<L, S>), dim3(1), dim3(T1D), lds_bytes, s, G, rp, A, dctrl.p, budget, lds_arg)
         if (cfg->random_choice) {
             HFV_L1D(RANDOM_CHOICE, EXACT);
         } else if (cfg->first_order) {
