@@ -99,6 +99,28 @@ def sod_run(lib, n=1000):
     return out
 
 
+def copy_bandwidth(stream, mib=1024, reps=5):
+    """Achievable HBM bandwidth: a device-to-device copy of `mib` MiB on the
+    bench's stream, timed with HIP events; bytes read plus bytes written per
+    second, best of `reps`."""
+    import torch
+    src = torch.ones(mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    best = float("inf")
+    with torch.cuda.stream(stream):
+        dst.copy_(src)  # warm-up
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            dst.copy_(src)
+            e1.record(stream)
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+    del src, dst
+    return 2 * mib * (1 << 20) / (best / 1e3) / 1e9
+
+
 def cpu_baseline_sod(n=1000):
     """The reference's serial build (oracle/_ref) on the same Sod run, beside
     the device's: part of the CPU-baseline leg."""
@@ -268,6 +290,13 @@ def main():
         log(f"calibration copy: {args.calibrate_copy_mib} MiB read + {args.calibrate_copy_mib} MiB written")
         del src, dst
 
+    copy_gbs = None
+    try:
+        copy_gbs = copy_bandwidth(stream)
+        log(f"device copy: {copy_gbs:.0f} GB/s (read + write)")
+    except Exception as e:  # the main figure stands without it
+        log(f"copy bandwidth failed: {e!r}")
+
     cpu = None
     if not args.no_cpu_baseline:
         sample_steps = 10
@@ -336,7 +365,8 @@ def main():
                    "boundary": "periodic", "limiter": "monotonized_central", "riemann_solver": "hllc",
                    "cfl": 0.8, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "achievable_copy_gbs": copy_gbs},
         "compute_bound": compute_bound(ms_per_step),
         "cpu_baseline": cpu,
         "ensemble": ens,
