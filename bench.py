@@ -206,7 +206,17 @@ def cpu_baseline(n: int, t_sample: float, steps_sample: int):
     value = n * n * steps / sec
     return {"value": value, "unit": "cell-updates/s", "cores": threads, "kind": kind,
             "sample": f"KH {n}x{n}, {steps} steps (the 10 CFL-shrink steps), {sec:.1f} s wall, "
-                      f"OMP_NUM_THREADS={threads}"}
+                      f"OMP_NUM_THREADS={threads}",
+            "cpu": host_cpu()}
+
+
+def host_cpu():
+    """Model name and logical CPU count of the host, from /proc/cpuinfo."""
+    try:
+        names = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")]
+        return f"{names[0]} ({len(names)} logical CPUs)" if names else None
+    except OSError:
+        return None
 
 
 def main():
