@@ -80,3 +80,30 @@ def test_failed_solve_is_reported(gpulib):
         sim.launch("muscl_hancock_2d", "riemann_solver_exact", "minmod", 0.5, tf=1.0)
     assert e.value.code == 1
     assert "Bisection method did not converge" in e.value.text
+
+
+def test_sedov_taylor_shock_radius(gpulib):
+    """The 2D exact solver on the Sedov blast (256^2 to t = 0.05, the
+    workload's physics): the angle-averaged density peak lies at the
+    Sedov-Taylor radius R = xi0 (E t^2 / rho)^(1/4), with xi0 = 1.004 for
+    gamma = 1.4 in 2D. That value follows from the reference's similarity
+    table: r_s = 0.75 at t = 1 for E0 = 0.311357 (plot_sedov_blast_2d.py:125,
+    sedov_blast_2d.c:58). The peak trails the shock by about a cell (the oracle: -2.3% at
+    this resolution), so 4% is allowed."""
+    n = 256
+    sim = K.setup_sedov(gpulib, n)
+    res = sim.launch("muscl_hancock_2d", "riemann_solver_exact", "van_leer", 0.5, 0.05, tol=1e-8)
+    assert res.t == 0.05
+    W, U = K.state(sim)
+    g = sim.g
+    rho = W[0][g:g + n, g:g + n]
+    xc, yc = sim.mid_x[g:g + n], sim.mid_y[g:g + n]
+    rad = np.hypot(*np.meshgrid(xc, yc))
+    dx = xc[1] - xc[0]
+    bins = np.arange(0.0, 0.6, dx)
+    idx = np.digitize(rad.ravel(), bins)
+    prof = np.array([rho.ravel()[idx == i].mean() if np.any(idx == i) else 0.0 for i in range(1, len(bins))])
+    r_peak = (bins[:-1] + 0.5 * dx)[np.argmax(prof)]
+    R = 1.004 * (1.0 * 0.05 ** 2 / 1.0) ** 0.25
+    print(f"Sedov 256^2: density peak at r = {r_peak:.4f}, Sedov-Taylor R = {R:.4f} ({(r_peak - R) / R:+.2%})")
+    assert abs(r_peak - R) / R < 0.04
