@@ -74,28 +74,58 @@ __device__ __forceinline__ int map_index(int c, int n, int g, int bc_lo, int bc_
     return c;
 }
 
-// Primitive state of cell (row q, column c) as the reference's stencil sees it.
+// A cell's stored values as loaded, before conversion: the conserved state
+// of the cell the boundary mapping names (the cell itself inside the grid)
+// and the velocity flips, or (WMODE) the primitive state itself.
+struct RawCell {
+    double v[4];
+    int fu, fv;
+};
+
 template <bool WMODE>
-__device__ __forceinline__ void load_prim(const Grid2D& G, const Fields4& U, const Fields4& W, int q, int c,
-                                          double* w)
+__device__ __forceinline__ void load_raw(const Grid2D& G, const Fields4& U, const Fields4& W, int q, int c,
+                                         RawCell& rc)
 {
     q = min(max(q, 0), G.ty - 1);
     c = min(max(c, 0), G.tx - 1);
     if constexpr (WMODE) {
         const long long k = (long long)q * G.pitch + c;
-        w[0] = W.f[0][k];
-        w[1] = W.f[1][k];
-        w[2] = W.f[2][k];
-        w[3] = W.f[3][k];
+#pragma unroll
+        for (int f = 0; f < 4; f++) rc.v[f] = W.f[f][k];
+        rc.fu = rc.fv = 0;
     } else {
-        int fu, fv;
-        const int cs = map_index(c, G.nx, G.g, G.bc[0], G.bc[1], &fu);
-        const int qs = map_index(q, G.ny, G.g, G.bc[2], G.bc[3], &fv);
+        const int cs = map_index(c, G.nx, G.g, G.bc[0], G.bc[1], &rc.fu);
+        const int qs = map_index(q, G.ny, G.g, G.bc[2], G.bc[3], &rc.fv);
         const long long k = (long long)qs * G.pitch + cs;
-        c2p_2d(G.gamma, G.volume, U.f[0][k], U.f[1][k], U.f[2][k], U.f[3][k], w);
-        if (fu) w[1] = -w[1];
-        if (fv) w[2] = -w[2];
+#pragma unroll
+        for (int f = 0; f < 4; f++) rc.v[f] = U.f[f][k];
     }
+}
+
+// Primitive state from loaded values: c2p (system.c:763-802) and the
+// boundary flips.
+template <bool WMODE>
+__device__ __forceinline__ void raw_to_prim(const Grid2D& G, const RawCell& rc, double* w)
+{
+    if constexpr (WMODE) {
+#pragma unroll
+        for (int f = 0; f < 4; f++) w[f] = rc.v[f];
+    } else {
+        c2p_2d(G.gamma, G.volume, rc.v[0], rc.v[1], rc.v[2], rc.v[3], w);
+        if (rc.fu) w[1] = -w[1];
+        if (rc.fv) w[2] = -w[2];
+    }
+}
+
+// Primitive state of cell (row q, column c) as the reference's stencil sees it.
+template <bool WMODE>
+__device__ __forceinline__ void load_prim(const Grid2D& G, const Fields4& U, const Fields4& W, int q, int c,
+                                          double* w, RawCell* keep = nullptr)
+{
+    RawCell rc;
+    load_raw<WMODE>(G, U, W, q, c, rc);
+    raw_to_prim<WMODE>(G, rc, w);
+    if (keep) *keep = rc;
 }
 
 // y-predictor of the cell whose own state is wc, with wb below and wt above.
@@ -385,10 +415,16 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     double wa[4], wb[4], wc[4];
     double a_row = 0.0, a_next = 0.0;  // sound speeds of rows r and r+1 (MUSCL)
     double yt[4], fyb[4];
+    // The conserved state loaded for rows r and r+1 (a row's c2p reads it two
+    // iterations before the row's update, which then needs no second load:
+    // inside the grid the boundary mapping is the identity), and the next
+    // row's, loaded an iteration ahead.  WMODE loads primitive states, so its
+    // updates load U.
+    RawCell raw_a, raw_b, next;
     if constexpr (LIM == FIRST_ORDER) {
         double wm1[4];
         load_prim<WMODE>(G, U, W, y0 - 1, col, wm1);
-        load_prim<WMODE>(G, U, W, y0, col, wa);
+        load_prim<WMODE>(G, U, W, y0, col, wa, &raw_a);
 #pragma unroll
         for (int f = 0; f < 4; f++) {
             wb[f] = wa[f];  // unused
@@ -405,8 +441,8 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         double wm2[4], wm1[4];
         load_prim<WMODE>(G, U, W, y0 - 2, col, wm2);
         load_prim<WMODE>(G, U, W, y0 - 1, col, wm1);
-        load_prim<WMODE>(G, U, W, y0, col, wa);
-        load_prim<WMODE>(G, U, W, y0 + 1, col, wb);
+        load_prim<WMODE>(G, U, W, y0, col, wa, &raw_a);
+        load_prim<WMODE>(G, U, W, y0 + 1, col, wb, &raw_b);
         double lo[4], hi_m1[4], lo0[4];
         ypredict<LIM>(wm2, wm1, wa, cy, gamma, lo, hi_m1);         // row y0-1: top face state
         a_row = ypredict<LIM>(wm1, wa, wb, cy, gamma, lo0, yt);    // row y0: bottom and top face states
@@ -431,20 +467,23 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         V.f[3][k] = U.f[3][k] - fyb[3] * dt * G.area_y;
     }
 
+    constexpr int AHEAD = (LIM == FIRST_ORDER) ? 1 : 2;  // the row a step's iteration r reads
+    load_raw<WMODE>(G, U, W, y0 + AHEAD, col, next);
     for (int r = y0; r < y1; r++) {
         // y: row r+1's face states (the flux through the top face of row r
         // follows below); x: the face states of row r, with the lane's
         // neighbours taken by shuffles
         double ylo[4], yhi[4], xlo[4], xhi[4];
+        const RawCell raw_c = next;  // row r + AHEAD
+        raw_to_prim<WMODE>(G, raw_c, wc);
+        if (r + 1 < y1) load_raw<WMODE>(G, U, W, r + 1 + AHEAD, col, next);
         if constexpr (LIM == FIRST_ORDER) {
-            load_prim<WMODE>(G, U, W, r + 1, col, wc);
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 ylo[f] = yhi[f] = wc[f];
                 xlo[f] = xhi[f] = wa[f];
             }
         } else {
-            load_prim<WMODE>(G, U, W, r + 2, col, wc);
             a_next = ypredict<LIM>(wa, wb, wc, cy, gamma, ylo, yhi);
             double wl[4], wr[4];
 #pragma unroll
@@ -555,7 +594,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             double u[4];
 #pragma unroll
             for (int f = 0; f < 4; f++) {
-                double x = U.f[f][k];
+                double x = WMODE ? U.f[f][k] : raw_a.v[f];
                 x += fxl[f] * dt * G.area_x;
                 x -= fxr[f] * dt * G.area_x;
                 x += fyb[f] * dt * G.area_y;
@@ -598,6 +637,8 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             yt[f] = yhi[f];
             fyb[f] = fyt[f];
         }
+        raw_a = (LIM == FIRST_ORDER) ? raw_c : raw_b;
+        raw_b = raw_c;
         a_row = a_next;
         hyb = hy;
     }
