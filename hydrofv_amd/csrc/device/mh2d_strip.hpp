@@ -270,8 +270,26 @@ __device__ __forceinline__ void tsrs_compact(double gamma, const HllcFace (&F)[2
     }
 }
 
-__device__ __forceinline__ double shfl_from_left(double v) { return __shfl_up(v, 1, LANES); }
-__device__ __forceinline__ double shfl_from_right(double v) { return __shfl_down(v, 1, LANES); }
+// Neighbour exchange across the wavefront by the GFX9 whole-wavefront DPP
+// shifts (v_mov_b32_dpp wave_shr:1 / wave_shl:1), register to register, two
+// 32-bit moves per double: lane i receives lane i-1's (from_left) or lane
+// i+1's (from_right) value; the edge lane with no neighbour keeps its own,
+// as with __shfl_up / __shfl_down over the 64 lanes.
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = (int)b, hi = (int)(b >> 32);
+    const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+    const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi2 << 32) | (unsigned int)lo2);
+}
+
+__device__ __forceinline__ double shfl_from_left(double v) { return dpp_shift<DPP_WAVE_SHR1>(v); }
+__device__ __forceinline__ double shfl_from_right(double v) { return dpp_shift<DPP_WAVE_SHL1>(v); }
 
 // DEFER (exact solver): a face whose two states differ (its Riemann problem
 // needs the pressure iteration, or the vacuum sampler) is not solved by the
