@@ -149,9 +149,13 @@ static int attach_device_bc(const BoundaryConditionParam *bc, hfv_mh2d *h)
     DeviceBc found[4];
     for (int s = 0; s < 4; s++)
         if (flags[s] == BOUNDARY_CONDITION_CUSTOM && !find_device_bc(hooks[s], &found[s])) return 0;
-    for (int s = 0; s < 4; s++)
-        if (flags[s] == BOUNDARY_CONDITION_CUSTOM && hfv_mh2d_set_device_bc(h, s, found[s].device, found[s].user) != HFV_OK)
+    for (int s = 0; s < 4; s++) {
+        if (flags[s] == BOUNDARY_CONDITION_CUSTOM && hfv_mh2d_set_device_bc(h, s, found[s].device, found[s].user) != HFV_OK) {
+            for (int k = 0; k < s; k++)  /* all or none: back to the host path */
+                if (flags[k] == BOUNDARY_CONDITION_CUSTOM) (void)hfv_mh2d_set_device_bc(h, k, NULL, NULL);
             return 0;
+        }
+    }
     return 1;
 }
 
