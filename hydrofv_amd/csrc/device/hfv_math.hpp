@@ -64,10 +64,74 @@ HFV_HD __forceinline__ double limit_rt(int lim, double a, double b)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Square roots of groups of arguments.  sqrt() of a double compiles for
+// gfx950 to a reciprocal square root estimate refined by two Goldschmidt
+// steps (ten instructions), wrapped in a scaling by 2^256 of arguments below
+// 2^-767 and a fix-up that passes 0 and +inf through (about ten more).  For an
+// argument in [2^-767, +inf) the wrapping leaves the refined value unchanged,
+// so sqrt_core, the refinement alone in the same operation order, returns the
+// bits of sqrt() there: the correctly rounded root.  sqrt_rn takes the core
+// for a group of arguments when every argument on every lane of the
+// wavefront lies in that range, and sqrt() for the whole group otherwise.
+HFV_HD __forceinline__ bool sqrt_core_ok(double x) { return (x >= 0x1p-767) & (x < INFINITY); }
+
+HFV_HD __forceinline__ double sqrt_core(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    const double s0 = x * y;
+    const double h0 = y * 0.5;
+    const double r0 = fma(-h0, s0, 0.5);
+    const double h1 = fma(h0, r0, h0);
+    const double s1 = fma(s0, r0, s0);
+    const double d0 = fma(-s1, s1, x);
+    const double s2 = fma(d0, h1, s1);
+    const double d1 = fma(-s2, s2, x);
+    return fma(d1, h1, s2);
+#else
+    return sqrt(x);  // host builds (CPU tests): correctly rounded as well
+#endif
+}
+
+// ok on every active lane of the wavefront (host builds: ok)
+HFV_HD __forceinline__ bool wave_all(bool ok)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ballot_w64(!ok) == 0;
+#else
+    return ok;
+#endif
+}
+
+// x[k] = sqrt(x[k]) for k < K, bit for bit
+template <int K>
+HFV_HD __forceinline__ void sqrt_rn(double (&x)[K])
+{
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < K; k++) ok = ok & sqrt_core_ok(x[k]);
+    if (wave_all(ok)) {
+#pragma unroll
+        for (int k = 0; k < K; k++) x[k] = sqrt_core(x[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) x[k] = sqrt(x[k]);
+    }
+}
+
 // utils.c:23-26
 HFV_HD __forceinline__ double sound_speed(double gamma, double rho, double p)
 {
     return sqrt(gamma * p / rho);
+}
+
+// sound_speed of a lane's cell, with the root taken by sqrt_rn
+HFV_HD __forceinline__ double sound_speed_rn(double gamma, double rho, double p)
+{
+    double x[1] = {gamma * p / rho};
+    sqrt_rn<1>(x);
+    return x[0];
 }
 
 // riemann_solver.c:144-152
@@ -331,13 +395,20 @@ template <int N>
 HFV_HD __forceinline__ void hllc_guess_begin(double gamma, HllcFace (&F)[N], HllcGuess<N>& G)
 {
     const double h = 0.5 * (gamma - 1.0);
+    // the faces' sound speeds (utils.c:23-26), roots taken together
+    double a2[2 * N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        a2[2 * i] = gamma * F[i].wl[3] / F[i].wl[0];
+        a2[2 * i + 1] = gamma * F[i].wr[3] / F[i].wr[0];
+    }
+    sqrt_rn<2 * N>(a2);
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const double rl = F[i].wl[0], ul = F[i].wl[1], pl = F[i].wl[3];
         const double rr = F[i].wr[0], ur = F[i].wr[1], pr = F[i].wr[3];
-        const double al_ = sound_speed(gamma, rl, pl), ar_ = sound_speed(gamma, rr, pr);
-        const double al = (rl > 0.0) ? al_ : 0.0;
-        const double ar = (rr > 0.0) ? ar_ : 0.0;
+        const double al = (rl > 0.0) ? a2[2 * i] : 0.0;
+        const double ar = (rr > 0.0) ? a2[2 * i + 1] : 0.0;
         F[i].al = al;
         F[i].ar = ar;
         G.vac[i] = hllc_is_vacuum(gamma, rl, ul, al, rr, ur, ar);
@@ -357,8 +428,9 @@ HFV_HD __forceinline__ void hllc_guess_begin(double gamma, HllcFace (&F)[N], Hll
 HFV_HD __forceinline__ double tsrs_guess(double gamma, double rl, double pl, double rr, double pr, double du,
                                          double ppv)
 {
-    const double gl = sqrt(coeff_A(gamma, rl) / (ppv + coeff_B(gamma, pl)));
-    const double gr = sqrt(coeff_A(gamma, rr) / (ppv + coeff_B(gamma, pr)));
+    double g2[2] = {coeff_A(gamma, rl) / (ppv + coeff_B(gamma, pl)), coeff_A(gamma, rr) / (ppv + coeff_B(gamma, pr))};
+    sqrt_rn<2>(g2);
+    const double gl = g2[0], gr = g2[1];
     return (gl * pl + gr * pr - du) / (gl + gr);
 }
 
@@ -414,14 +486,19 @@ HFV_HD __forceinline__ void hllc_end_regular_faces(double gamma, const HllcFace 
 {
     // hllc_q (hllc.c:30-40) on both sides
     const double kq = 0.5 + 0.5 / gamma;
-    double ql[N], qr[N];
+    double ql[N], qr[N], q2[2 * N];
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const double ps = F[i].ps, pl = F[i].wl[3], pr = F[i].wr[3];
-        const double qls = sqrt(1.0 + kq * (ps / pl - 1.0));
-        const double qrs = sqrt(1.0 + kq * (ps / pr - 1.0));
-        ql[i] = (ps <= pl) ? 1.0 : qls;
-        qr[i] = (ps <= pr) ? 1.0 : qrs;
+        q2[2 * i] = 1.0 + kq * (ps / pl - 1.0);
+        q2[2 * i + 1] = 1.0 + kq * (ps / pr - 1.0);
+    }
+    sqrt_rn<2 * N>(q2);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const double ps = F[i].ps, pl = F[i].wl[3], pr = F[i].wr[3];
+        ql[i] = (ps <= pl) ? 1.0 : q2[2 * i];
+        qr[i] = (ps <= pr) ? 1.0 : q2[2 * i + 1];
     }
 #pragma unroll
     for (int i = 0; i < N; i++) {

@@ -72,8 +72,9 @@ HFV_HD __forceinline__ const double* pow_exp_tab()
 #endif
 }
 
-// log x as a double-double, x positive finite normal
-HFV_HD __forceinline__ DD log_dd(double x)
+// log x as a double-double, x positive finite normal; lt: the log table
+// (POW_LOG_TAB_*, or a copy of it)
+HFV_HD __forceinline__ DD log_dd(double x, const double* lt = pow_log_tab())
 {
     int k;
     double m = frexp(x, &k);  // m in [0.5, 1)
@@ -82,7 +83,7 @@ HFV_HD __forceinline__ DD log_dd(double x)
         k -= 1;
     }
     const int i = (int)(m * 64.0) - 48;  // m in [0.75 + i/64, 0.75 + (i+1)/64)
-    const double* t = pow_log_tab() + 3 * i;
+    const double* t = lt + 3 * i;
     const double c = t[0];
     // r = m c - 1 exactly: ph + pl = m c, ph - 1 exact (ph in [0.5, 2])
     const double ph = m * c;
@@ -106,8 +107,8 @@ HFV_HD __forceinline__ DD log_dd(double x)
     return fast_two_sum(s2.hi, low);
 }
 
-// exp(w.hi + w.lo) for |w.hi| < 700
-HFV_HD __forceinline__ double exp_dd(DD w)
+// exp(w.hi + w.lo) for |w.hi| < 700; et: the exp table (POW_EXP_TAB_*, or a copy)
+HFV_HD __forceinline__ double exp_dd(DD w, const double* et = pow_exp_tab())
 {
     const double nd = rint(w.hi * POW_64_OVER_LN2);
     const int n = (int)nd;
@@ -124,7 +125,7 @@ HFV_HD __forceinline__ double exp_dd(DD w)
     q = fma(q, rh, 1.0 / 6.0);
     q = fma(q, rh, 1.0 / 2.0);
     const double b = fma(rl, rh, rl) + (rh * rh) * q;
-    const double* t = pow_exp_tab() + 2 * j;
+    const double* t = et + 2 * j;
     const double th = t[0], tl = t[1];
     // 2^(j/64) (1 + rh + b) = th + th rh + (th b + tl (1 + rh + b))
     const double p = th * rh;
@@ -135,16 +136,16 @@ HFV_HD __forceinline__ double exp_dd(DD w)
 }
 
 // The logarithm pow_pos takes of x: log |x| for finite nonzero x, else 0.
-HFV_HD __forceinline__ DD pow_log(double x)
+HFV_HD __forceinline__ DD pow_log(double x, const double* lt = pow_log_tab())
 {
     const double ax = fabs(x);
     const bool regular = (ax > 0.0) & (ax < INFINITY);  // false for 0, inf, NaN
-    return log_dd(regular ? ax : 1.0);
+    return log_dd(regular ? ax : 1.0, lt);
 }
 
 // x^y given l = pow_log(x): the rest of pow_pos, so that powers of one base
 // share its logarithm with the same bits as separate pow_pos calls.
-HFV_HD __forceinline__ double pow_with_log(double x, double y, DD l)
+HFV_HD __forceinline__ double pow_with_log(double x, double y, DD l, const double* et = pow_exp_tab())
 {
     const double ax = fabs(x);
     const bool regular = (ax > 0.0) & (ax < INFINITY);
@@ -154,7 +155,7 @@ HFV_HD __forceinline__ double pow_with_log(double x, double y, DD l)
     const bool ovf = wh > 709.782712893384;
     const bool unf = wh < -745.2;
     const bool inrange = !(ovf | unf) & (wh == wh);
-    double r = exp_dd(DD{inrange ? wh : 0.0, inrange ? wl : 0.0});
+    double r = exp_dd(DD{inrange ? wh : 0.0, inrange ? wl : 0.0}, et);
     // a positive finite x other than 1 and a nonzero y with x^y in range:
     // every special case below leaves r as it is
     if (regular & (x > 0.0) & inrange & (x != 1.0) & (y != 0.0)) return r;
@@ -178,5 +179,12 @@ HFV_HD __forceinline__ double pow_with_log(double x, double y, DD l)
 
 // x^y for all x, y (C99 pow semantics; accuracy as stated at the top)
 HFV_HD __forceinline__ double pow_pos(double x, double y) { return pow_with_log(x, y, pow_log(x)); }
+
+// pow_pos with the log and exp tables read from lt and et (copies of
+// POW_LOG_TAB_* and POW_EXP_TAB_*, e.g. in LDS); the same bits
+HFV_HD __forceinline__ double pow_pos(double x, double y, const double* lt, const double* et)
+{
+    return pow_with_log(x, y, pow_log(x, lt), et);
+}
 
 }  // namespace hfv

@@ -411,7 +411,9 @@ __global__ void __launch_bounds__(LANES) k_probe_flux4_wave(double gamma, double
                                                             const double* wl, const double* wr, double* f, int* err)
 {
     __shared__ double q[6 * LANES];
+    __shared__ double pow_lds[POW_TABS_LEN];
     const int lane = threadIdx.x;
+    stage_pow_tables(pow_lds, lane);
     const long long i = (long long)blockIdx.x * LANES + lane;
     const long long ia = (i < half) ? i : 0, ib = (half + i < n) ? half + i : 0;
     double la[4], ra[4], lb[4], rb[4];
@@ -424,7 +426,7 @@ __global__ void __launch_bounds__(LANES) k_probe_flux4_wave(double gamma, double
     HllcFace A, B;
     hllc_begin(gamma, tol, la, ra, A);
     hllc_begin(gamma, tol, lb, rb, B);
-    trrs_powers(gamma, tol, A, B, q, lane);
+    trrs_powers(gamma, tol, A, B, q, lane, pow_lds);
     double oa[4], ob[4];
     int ea = hllc_end(gamma, A, oa);
     int eb = hllc_end(gamma, B, ob);
@@ -453,6 +455,23 @@ __global__ void k_probe_pow(int impl, long long n, const double* x, const double
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     out[i] = impl ? pow_pos(x[i], y[i]) : pow(x[i], y[i]);
+}
+
+// impl 0: sqrt(); 1: sqrt_rn over the wavefront (one argument per lane);
+// 2: sqrt_core alone, for arguments in [2^-767, +inf) only.
+__global__ void __launch_bounds__(LANES) k_probe_sqrt(int impl, long long n, const double* x, double* out)
+{
+    const long long i = (long long)blockIdx.x * LANES + threadIdx.x;
+    const double v = x[(i < n) ? i : 0];
+    double r;
+    if (impl == 1) {
+        double g[1] = {v};
+        sqrt_rn<1>(g);
+        r = g[0];
+    } else {
+        r = (impl == 2) ? sqrt_core(v) : sqrt(v);
+    }
+    if (i < n) out[i] = r;
 }
 
 __global__ void k_probe_limit(int lim, long long n, const double* a, const double* b, double* out)
@@ -644,6 +663,20 @@ int hfv_probe_flux(int nf, int solver, double gamma, double tol, long long n, co
     if (hipMemcpy(f, df.p, m * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(err, de.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
         return HFV_EDEVICE;
+    return HFV_OK;
+}
+
+int hfv_probe_sqrt(int impl, long long n, const double* x, double* out)
+{
+    if (n < 0 || impl < 0 || impl > 2) return HFV_EINVAL;
+    if (n == 0) return HFV_OK;
+    DevBuf<double> dx, dout;
+    if (!dx.alloc((size_t)n) || !dout.alloc((size_t)n)) return HFV_ENOMEM;
+    if (hipMemcpy(dx.p, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return HFV_EDEVICE;
+    hipLaunchKernelGGL(k_probe_sqrt, dim3((unsigned)((n + LANES - 1) / LANES)), dim3(LANES), 0, 0, impl, n, dx.p,
+                       dout.p);
+    if (hipGetLastError() != hipSuccess) return HFV_EDEVICE;
+    if (hipMemcpy(out, dout.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return HFV_EDEVICE;
     return HFV_OK;
 }
 

@@ -137,7 +137,7 @@ __device__ __forceinline__ double ypredict(const double* wb, const double* wc, c
     double d[4];
 #pragma unroll
     for (int f = 0; f < 4; f++) d[f] = limit<LIM>(wc[f] - wb[f], wt[f] - wc[f]);
-    const double a = sound_speed(gamma, wc[0], wc[3]);
+    const double a = sound_speed_rn(gamma, wc[0], wc[3]);
     predict_y(wc, d, c, a, lo, hi);
     return a;
 }
@@ -160,6 +160,19 @@ __device__ __forceinline__ int yflux(double gamma, double tol, const double* hi,
     return e;
 }
 
+// pow_pos's log and exp tables copied into LDS (pow_lds: POW_TABS_LEN
+// doubles, log table first).  Read from LDS, a table lookup waits on the LDS
+// counter; read from global memory it would wait on the vector-memory
+// counter, which completes in order and so would also wait for the row loads
+// issued ahead of it.  Every lane of the one-wavefront workgroup calls this.
+constexpr int POW_LOG_TAB_LEN = 48 * 3, POW_TABS_LEN = 48 * 3 + 64 * 2;
+__device__ __forceinline__ void stage_pow_tables(double* pow_lds, int lane)
+{
+    for (int i = lane; i < POW_LOG_TAB_LEN; i += LANES) pow_lds[i] = POW_LOG_TAB_D[i];
+    for (int i = lane; i < POW_TABS_LEN - POW_LOG_TAB_LEN; i += LANES) pow_lds[POW_LOG_TAB_LEN + i] = POW_EXP_TAB_D[i];
+    __syncthreads();
+}
+
 // The powers of the two-rarefaction pressure guesses pending on two faces per
 // lane (riemann_solver.c:191-199), evaluated 64 at a time over the
 // wavefront: the lanes' pending bases are packed into LDS, each lane raises
@@ -170,7 +183,8 @@ __device__ __forceinline__ int yflux(double gamma, double tol, const double* hi,
 // cannot settle (about 1 in 1000 on a Kelvin-Helmholtz run), and its pass
 // is skipped when no lane has one.  q: 384 doubles of LDS; the workgroup is
 // this one wavefront, and every lane must call this.
-__device__ __forceinline__ void trrs_powers(double gamma, double tol, HllcFace& A, HllcFace& B, double* q, int lane)
+__device__ __forceinline__ void trrs_powers(double gamma, double tol, HllcFace& A, HllcFace& B, double* q, int lane,
+                                            const double* pow_lds)
 {
     const bool ta = A.kind == HLLC_TRRS, tb = B.kind == HLLC_TRRS;
     const unsigned long long ma = __ballot(ta), mb = __ballot(tb);
@@ -187,7 +201,7 @@ __device__ __forceinline__ void trrs_powers(double gamma, double tol, HllcFace& 
     const double e1 = h / gamma;
     for (int b = 0; b < 2 * n; b += LANES) {
         const int i = b + lane;
-        if (i < 2 * n) q[i] = pow_pos(q[i], e1);
+        if (i < 2 * n) q[i] = pow_pos(q[i], e1, pow_lds, pow_lds + POW_LOG_TAB_LEN);
     }
     __syncthreads();
     double xa = 0.0, xb = 0.0;
@@ -205,7 +219,7 @@ __device__ __forceinline__ void trrs_powers(double gamma, double tol, HllcFace& 
     const double e2 = gamma / h;
     for (int b = 0; b < m; b += LANES) {
         const int i = b + lane;
-        if (i < m) q2[i] = pow_pos(q2[i], e2);
+        if (i < m) q2[i] = pow_pos(q2[i], e2, pow_lds, pow_lds + POW_LOG_TAB_LEN);
     }
     __syncthreads();
     if (la) {
@@ -428,6 +442,8 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     long long err_cell = -1;
     double smx = -INFINITY, smy = -INFINITY;
     __shared__ double trrs_q[6 * LANES];
+    __shared__ double pow_lds[POW_TABS_LEN];
+    if constexpr (SOLVER == HLLC) stage_pow_tables(pow_lds, lane);
 
     // ---- prologue: rows y0-2 .. y0+1 (first order: y0-1 .. y0) ----------
     double wa[4], wb[4], wc[4];
@@ -536,7 +552,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             if constexpr (BRANCHY) {
                 hllc_begin_branchy(gamma, tol, F[0]);
                 hllc_begin_branchy(gamma, tol, F[1]);
-                trrs_powers(gamma, tol, F[0], F[1], trrs_q, lane);
+                trrs_powers(gamma, tol, F[0], F[1], trrs_q, lane, pow_lds);
                 ey = hllc_end_branchy(gamma, F[0], o[0]);
                 ex = hllc_end_branchy(gamma, F[1], o[1]);
             } else {
@@ -544,7 +560,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
                 hllc_guess_begin<2>(gamma, F, Gs);
                 tsrs_compact(gamma, F, Gs, trrs_q, lane);
                 hllc_guess_end<2>(tol, F, Gs);
-                trrs_powers(gamma, tol, F[0], F[1], trrs_q, lane);
+                trrs_powers(gamma, tol, F[0], F[1], trrs_q, lane, pow_lds);
                 // both faces' regular fluxes together, then the rare vacuum faces
                 int e2[2];
                 hllc_end_regular_faces<2>(gamma, F, o, e2);
@@ -635,7 +651,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             // CFL maxima of the new state (utils.c:75-92)
             double w[4];
             c2p_2d(gamma, G.volume, u[0], u[1], u[2], u[3], w);
-            const double a = sound_speed(gamma, w[0], w[3]);
+            const double a = sound_speed_rn(gamma, w[0], w[3]);
             const double sx_ = a + fabs(w[1]);
             const double sy_ = a + fabs(w[2]);
             if (sx_ > smx) smx = sx_;

@@ -88,6 +88,7 @@ PROTOTYPES = {
                                  _P(C.c_int)]),
     "hfv_probe_limit": (C.c_int, [C.c_int, C.c_longlong, dp, dp, dp]),
     "hfv_probe_pow": (C.c_int, [C.c_int, C.c_longlong, dp, dp, dp]),
+    "hfv_probe_sqrt": (C.c_int, [C.c_int, C.c_longlong, dp, dp]),
     "hfv_device_error_string": (C.c_char_p, []),
     "hfv_muscl_hancock_1d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
     "hfv_muscl_hancock_2d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
@@ -148,6 +149,16 @@ def probe_pow(lib, impl: int, x: np.ndarray, y: np.ndarray):
     y = np.ascontiguousarray(y, dtype=np.float64)
     out = np.zeros_like(x)
     check(lib, lib.hfv_probe_pow(impl, x.size, _ptr(x), _ptr(y), _ptr(out)), "hfv_probe_pow")
+    return out
+
+
+
+def probe_sqrt(lib, impl: int, x: np.ndarray):
+    """sqrt(x) on the device: impl 0 the library sqrt, 1 the kernels' grouped
+    root, 2 its unscaled core (arguments in [2^-767, +inf) only)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    check(lib, lib.hfv_probe_sqrt(impl, x.size, _ptr(x), _ptr(out)), "hfv_probe_sqrt")
     return out
 
 

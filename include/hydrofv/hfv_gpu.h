@@ -230,6 +230,12 @@ int hfv_probe_flux(int nf, int solver, double gamma, double tol, long long n, co
 /* out[n] = x[n]^y[n]: impl 0 the device library pow, impl 1 the kernels'
  * pow of the two-rarefaction guess (riemann_solver.c:191-199). */
 int hfv_probe_pow(int impl, long long n, const double *x, const double *y, double *out);
+/* out[n] = sqrt(x[n]) as the kernels take it: impl 0 the device library sqrt, impl 1 the
+ * kernels' grouped root (library sequence without its range scaling when every argument of
+ * the wavefront is in [2^-767, +inf)), impl 2 that unscaled core alone (for x[n] in range).
+ * The kernels' roots of sound speeds and HLLC wave-speed factors (utils.c:23-26,
+ * riemann_solver_hllc.c:30-40) must equal impl 0 bit for bit. */
+int hfv_probe_sqrt(int impl, long long n, const double *x, double *out);
 /* out[n] = limit_slope(a[n], b[n]) for limiter 1/2/3 (slope_limiter.c:93-115). */
 int hfv_probe_limit(int limiter, long long n, const double *a, const double *b, double *out);
 

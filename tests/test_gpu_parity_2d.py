@@ -240,6 +240,39 @@ def test_hllc_invalid_wave_speed_error(gpulib):
     assert "Invalid wave speed for the HLLC riemann solver" in e.value.text
 
 
+def test_probe_sqrt_grouped_root_is_the_library_root(gpulib):
+    """The kernels' square roots (sqrt_rn: the library's refinement without its
+    range scaling when every argument of a wavefront is in [2^-767, +inf))
+    give the bits of the device library sqrt and of the correctly rounded
+    root, for wavefronts entirely in range and for wavefronts holding an
+    argument outside it (0, denormals, values just below 2^-767, +inf, NaN,
+    negatives), which take the library sequence."""
+    r = np.random.default_rng(11)
+    n = 64 * 4096
+    # bit patterns uniform over [2^-767, max finite]
+    lo = np.float64(2.0 ** -767).view(np.int64)
+    hi = np.float64(np.finfo(np.float64).max).view(np.int64)
+    inr = r.integers(lo, hi, size=n, endpoint=True).view(np.float64)
+    inr[:8] = [2.0 ** -767, np.nextafter(2.0 ** -767, 1.0), 1.0, 2.0, 0.5, np.finfo(np.float64).max, 4.0, 1e-5]
+    # physical ranges the kernels see: gamma*p/rho and 1 + kq*(ps/p - 1)
+    phys = np.exp(r.uniform(np.log(1e-12), np.log(1e12), n))
+    special = np.array([0.0, -0.0, 5e-324, 2.0 ** -1022, np.nextafter(2.0 ** -767, 0.0), 2.0 ** -800, np.inf,
+                        np.nan, -1.0, -np.inf, 1e-310])
+    # wavefronts that mix one out-of-range argument into in-range ones
+    mixed = r.uniform(0.1, 10.0, 64 * len(special))
+    mixed[::64] = special
+    for x in (inr, phys, mixed, special):
+        lib = gpu.probe_sqrt(gpulib, 0, x)
+        grp = gpu.probe_sqrt(gpulib, 1, x)
+        with np.errstate(invalid="ignore"):
+            ref = np.sqrt(x)
+        np.testing.assert_array_equal(grp.view(np.int64), lib.view(np.int64))
+        np.testing.assert_array_equal(lib, ref)  # NaN positions compare equal
+    for x in (inr, phys):
+        core = gpu.probe_sqrt(gpulib, 2, x)
+        np.testing.assert_array_equal(core.view(np.int64), np.sqrt(x).view(np.int64))
+
+
 def test_probe_pow_pos_matches_host_build(gpulib):
     """The kernels' pow (pow_pos) gives the same bits on the
     device as its host build (tests/native), which test_adjoint_math checks
