@@ -491,6 +491,38 @@ void launch_hllc(hfv_mh2d* h, bool wmode, hipStream_t s)
     else launch_step_t<LIM, HLLC, false>(h, wmode, s);
 }
 
+// The step kernel a configuration launches (primitive-state mode aside,
+// which has the same register budget): for its occupancy.
+template <int LIM>
+const void* step_kernel_lim(const hfv_mh2d_config* c)
+{
+    if (c->riemann_solver == EXACT) return reinterpret_cast<const void*>(&k_mh2d_step_defer<LIM, false>);
+    if (c->hllc_form == HFV_HLLC_BRANCHES) return reinterpret_cast<const void*>(&k_mh2d_step<LIM, HLLC, false, true>);
+    return reinterpret_cast<const void*>(&k_mh2d_step<LIM, HLLC, false, false>);
+}
+
+const void* step_kernel(const hfv_mh2d_config* c)
+{
+    if (c->first_order) return step_kernel_lim<FIRST_ORDER>(c);
+    if (c->slope_limiter == MINMOD) return step_kernel_lim<MINMOD>(c);
+    if (c->slope_limiter == VAN_LEER) return step_kernel_lim<VAN_LEER>(c);
+    return step_kernel_lim<MC>(c);
+}
+
+// Step wavefronts the current device holds at once: CUs x resident
+// workgroups (one wavefront each) per CU; 0 when the runtime cannot say.
+long long resident_step_waves(const hfv_mh2d_config* c)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel(c), LANES, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return (long long)cus * per_cu;
+}
+
 void launch_step(hfv_mh2d* h, bool wmode, hipStream_t s)
 {
     const int lim = h->cfg.slope_limiter, sol = h->cfg.riemann_solver;
@@ -597,7 +629,7 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
     G.volume = cfg->volume;
     for (int s = 0; s < 4; s++) G.bc[s] = cfg->bc[s];
     G.nsx = (G.nx + STRIP_COLS - 1) / STRIP_COLS;
-    G.strip_rows = cfg->strip_rows > 0 ? cfg->strip_rows : default_strip_rows(G.nx, G.ny);
+    G.strip_rows = cfg->strip_rows > 0 ? cfg->strip_rows : balanced_strip_rows(G.nx, G.ny, resident_step_waves(cfg));
     G.nsy = (G.ny + G.strip_rows - 1) / G.strip_rows;
     RunParams& rp = h->rp;
     rp.cfl = cfg->cfl;

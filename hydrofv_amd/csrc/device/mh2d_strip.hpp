@@ -27,6 +27,7 @@
 #include "hfv_ctrl.hpp"
 #include "hfv_math.hpp"
 #include "hfv_pow.hpp"
+#include "hfv_strips.hpp"
 
 namespace hfv {
 
@@ -34,6 +35,7 @@ namespace hfv {
 constexpr int LANES = 64;
 constexpr int HALO = 2;
 constexpr int STRIP_COLS = LANES - 2 * HALO;  // 60 output columns per wave
+static_assert(STRIP_COLS == STRIP_COLS_2D, "strip width of hfv_strips.hpp");
 
 struct Fields4 {
     double* f[4];
@@ -714,18 +716,6 @@ __device__ __forceinline__ void strip_finish(StripResult r, Ctrl* out, int lane)
         const long long ec = __shfl(err_cell, src, LANES);
         if (lane == 0) report_error(out, ek, ec);
     }
-}
-
-// Strip height when the caller leaves it to the library: 32 rows (each
-// strip re-derives 4 halo rows, about 12% extra work at 32), halved down to
-// 8 while the grid has fewer than 4096 strips, so that a small grid still
-// puts several wavefronts on each of the chip's 1024 SIMDs.
-inline int default_strip_rows(int nx, int ny)
-{
-    const long long nsx = (nx + STRIP_COLS - 1) / STRIP_COLS;
-    int rows = 32;
-    while (rows > 8 && nsx * ((ny + rows - 1) / rows) < 4096) rows /= 2;
-    return rows;
 }
 
 // Consecutive workgroup ids are dealt round-robin to the 8 XCDs; remap them

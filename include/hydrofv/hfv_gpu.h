@@ -47,7 +47,8 @@ typedef struct hfv_mh2d_config {
     double tf;                     /* run while t < tf */
     long long max_steps;           /* stop after this many steps; < 0: no limit */
     double fixed_dt;               /* > 0: constant dt instead of the CFL dt */
-    int strip_rows;                /* rows per wavefront strip; 0: 32, fewer for small grids */
+    int strip_rows;                /* rows per wavefront strip; 0: the library's choice (28-40 rows,
+                                      fitted to the device's resident wavefronts; fewer for small grids) */
     int hllc_form;                 /* HFV_HLLC_SELECTS (0) or HFV_HLLC_BRANCHES (1), below */
     int first_order;               /* 1: first-order Godunov (godunov_first_order_2d,
                                       integrator_godunov_first_order.c:270-570): no

@@ -5,6 +5,7 @@
 // Test infrastructure only; built by tests/native/Makefile.
 #include "../../hydrofv_amd/csrc/device/hfv_adjoint.hpp"
 #include "../../hydrofv_amd/csrc/device/hfv_pow.hpp"
+#include "../../hydrofv_amd/csrc/device/hfv_strips.hpp"
 #include "hydrofv/hfv_gpu.h"
 
 #include <cstddef>
@@ -175,5 +176,9 @@ void hc_c2p_vjp(double gamma, double vol, long long n, const double* U, const do
         c2p_vjp(gamma, vol, u[0], w, mw + 4 * i, mu + 4 * i);
     }
 }
+
+// the 2D step's strip height for an nx x ny grid on a device holding `slots`
+// step wavefronts at once
+int hc_strip_rows(int nx, int ny, long long slots) { return hfv::balanced_strip_rows(nx, ny, slots); }
 
 }  // extern "C"

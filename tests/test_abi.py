@@ -169,6 +169,39 @@ def test_device_config_mirrors_match_the_header():
     assert out[7] == adjoint.Adj2DConfig.slope_limiter.offset
 
 
+def test_strip_height_fills_the_rounds_of_resident_wavefronts():
+    """The 2D step's default strip height (hfv_strips.hpp, host build in
+    tests/native): a large grid takes the height in [28, 40] whose strip
+    count best fills the rounds of resident wavefronts, small grids and an
+    unknown device keep the 32-rows-halved rule."""
+    import ctypes as C
+    import math
+    import os
+    so = os.path.join(os.path.dirname(os.path.abspath(__file__)), "native", "libhfv_hostcheck.so")
+    if not os.path.exists(so):
+        pytest.fail(f"{so} is not built (make -C tests/native)")
+    L = C.CDLL(so)
+    L.hc_strip_rows.argtypes = [C.c_int, C.c_int, C.c_longlong]
+    L.hc_strip_rows.restype = C.c_int
+
+    def strips(n, r):
+        return math.ceil(n / 60) * math.ceil(n / r)
+
+    def fill(t, slots):
+        return t / (math.ceil(t / slots) * slots)
+
+    # MI355X: 256 CUs x 8 resident one-wavefront workgroups
+    assert L.hc_strip_rows(4096, 4096, 2048) == 35          # 8142 strips: 99.4% of 4 rounds
+    assert L.hc_strip_rows(2048, 2048, 2048) == 16          # 35 x 128 strips: the halving rule
+    assert L.hc_strip_rows(4096, 4096, 0) == 32             # device unknown
+    assert L.hc_strip_rows(256, 256, 2048) == 8             # small grid: the halving rule
+    assert L.hc_strip_rows(1024, 1024, 2048) == 8
+    for n in (3000, 4096, 5000, 8192):
+        r = L.hc_strip_rows(n, n, 2048)
+        assert 28 <= r <= 40
+        assert fill(strips(n, r), 2048) >= max(fill(strips(n, q), 2048) for q in range(28, 41)) - 1e-3
+
+
 def test_device_bc_registry():
     """hfv_register_device_bc_2d: host-side table of device hook forms (no
     device work): NULL host hook refused, re-registration replaces, NULL
