@@ -31,21 +31,25 @@ inline double strip_round_fill(long long strips, long long slots)
     return (double)strips / ((double)rounds * (double)slots);
 }
 
-// The strip height when the caller leaves it to the library.  Where the
-// default is a full 32 rows, the height in [28, 40] whose launch fills its
-// rounds of resident wavefronts best (ties: the height nearest 32), e.g. 35
-// rows for 4096^2 on 2048 slots (8142 strips, 99.4% of four rounds, where 32
-// rows give 8832 strips, 86% of five).  slots <= 0 (unknown): the default.
+// The strip height when the caller leaves it to the library: the height near
+// default_strip_rows whose launch fills its rounds of resident wavefronts
+// best (ties: the height nearest the default).  Where the default is a full
+// 32 rows the heights tried are [28, 40], e.g. 35 rows for 4096^2 on 2048
+// slots (8142 strips, 99.4% of four rounds, where 32 rows give 8832 strips,
+// 86% of five); below that, the default +-25%, e.g. 18 rows for 2048^2
+// (3990 strips, 97% of two rounds, where 16 rows give 4480, 73% of three).
+// slots <= 0 (unknown): the default.
 inline int balanced_strip_rows(int nx, int ny, long long slots)
 {
     const int base = default_strip_rows(nx, ny);
-    if (base < 32 || slots <= 0) return base;
+    if (slots <= 0) return base;
+    const int lo = (base == 32) ? 28 : base - base / 4, hi = (base == 32) ? 40 : base + base / 4;
     const long long nsx = (nx + STRIP_COLS_2D - 1) / STRIP_COLS_2D;
     int best = base;
     double best_fill = strip_round_fill(nsx * ((ny + base - 1) / base), slots);
-    for (int r = 28; r <= 40; r++) {
+    for (int r = lo; r <= hi; r++) {
         const double f = strip_round_fill(nsx * ((ny + r - 1) / r), slots);
-        const int dr = r > 32 ? r - 32 : 32 - r, db = best > 32 ? best - 32 : 32 - best;
+        const int dr = r > base ? r - base : base - r, db = best > base ? best - base : base - best;
         if (f > best_fill + 1e-3 || (f >= best_fill - 1e-3 && dr < db)) {
             best = r;
             best_fill = f;

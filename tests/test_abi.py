@@ -171,9 +171,10 @@ def test_device_config_mirrors_match_the_header():
 
 def test_strip_height_fills_the_rounds_of_resident_wavefronts():
     """The 2D step's default strip height (hfv_strips.hpp, host build in
-    tests/native): a large grid takes the height in [28, 40] whose strip
-    count best fills the rounds of resident wavefronts, small grids and an
-    unknown device keep the 32-rows-halved rule."""
+    tests/native): the height near the 32-rows-halved rule (in [28, 40] for a
+    large grid, within 25% of the halved height for a smaller one) whose strip
+    count best fills the rounds of resident wavefronts; an unknown device
+    keeps the rule."""
     import ctypes as C
     import math
     import os
@@ -192,14 +193,21 @@ def test_strip_height_fills_the_rounds_of_resident_wavefronts():
 
     # MI355X: 256 CUs x 8 resident one-wavefront workgroups
     assert L.hc_strip_rows(4096, 4096, 2048) == 35          # 8142 strips: 99.4% of 4 rounds
-    assert L.hc_strip_rows(2048, 2048, 2048) == 16          # 35 x 128 strips: the halving rule
+    assert L.hc_strip_rows(2048, 2048, 2048) == 18          # 3990 strips, 97% of 2 rounds (16 rows: 73% of 3)
     assert L.hc_strip_rows(4096, 4096, 0) == 32             # device unknown
-    assert L.hc_strip_rows(256, 256, 2048) == 8             # small grid: the halving rule
-    assert L.hc_strip_rows(1024, 1024, 2048) == 8
+    assert L.hc_strip_rows(256, 256, 2048) == 6             # 215 strips: more, shorter strips
+    assert L.hc_strip_rows(1024, 1024, 2048) == 10          # 1854 strips in one round
+    assert L.hc_strip_rows(2048, 2048, 0) == 16
     for n in (3000, 4096, 5000, 8192):
         r = L.hc_strip_rows(n, n, 2048)
         assert 28 <= r <= 40
         assert fill(strips(n, r), 2048) >= max(fill(strips(n, q), 2048) for q in range(28, 41)) - 1e-3
+    for n in (600, 1024, 1500, 2048):
+        base = L.hc_strip_rows(n, n, 0)
+        r = L.hc_strip_rows(n, n, 2048)
+        window = range(base - base // 4, base + base // 4 + 1)
+        assert r in window
+        assert fill(strips(n, r), 2048) >= max(fill(strips(n, q), 2048) for q in window) - 1e-3
 
 
 def test_device_bc_registry():
