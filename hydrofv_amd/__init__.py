@@ -30,6 +30,15 @@ def load() -> C.CDLL:
             raise LibraryMissing(
                 f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
                 "or `make -C hydrofv_amd/csrc`")
+        # PyTorch for ROCm ships its own HIP runtime under the soname
+        # libamdhip64.so.7, which the library also links.  Imported first,
+        # torch's copy satisfies the library's dependency and the process
+        # holds one runtime; loaded the other way round, torch loads a second
+        # runtime that finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         abi.bind(lib)
         from . import ensemble, gpu

@@ -593,15 +593,15 @@ void fill_primitive(hfv_mh2d* h, hipStream_t s)
     h->w_filled = h->launches;
 }
 
-}  // namespace
-
-extern "C" {
-
-int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
+// Validation shared by the 2D entry points.  Fewer interior cells than ghost
+// cells in a direction is refused: the boundary mapping (map_index) and the
+// side fills need num_cells >= num_ghost_cells_side.
+int check_config(const hfv_mh2d_config* cfg)
 {
-    *out = nullptr;
     if (cfg->first_order != 0 && cfg->first_order != 1) return HFV_EINVAL;
     if (cfg->num_cells_x < 2 || cfg->num_cells_y < 2 || cfg->num_ghost_cells_side < (cfg->first_order ? 1 : 2))
+        return HFV_EINVAL;
+    if (cfg->num_cells_x < cfg->num_ghost_cells_side || cfg->num_cells_y < cfg->num_ghost_cells_side)
         return HFV_EINVAL;
     if (!cfg->first_order && (cfg->slope_limiter < 1 || cfg->slope_limiter > 3)) return HFV_EINVAL;
     if (cfg->riemann_solver != HLLC && cfg->riemann_solver != EXACT) return HFV_EINVAL;
@@ -611,16 +611,19 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
         if (cfg->bc[s] < BC_NONE || cfg->bc[s] > BC_CUSTOM) return HFV_EINVAL;
     if (cfg->strip_rows < 0) return HFV_EINVAL;
     if (cfg->hllc_form != HFV_HLLC_SELECTS && cfg->hllc_form != HFV_HLLC_BRANCHES) return HFV_EINVAL;
-    hfv_mh2d* h = new (std::nothrow) hfv_mh2d();
-    if (!h) return HFV_ENOMEM;
-    h->cfg = *cfg;
-    Grid2D& G = h->G;
+    return HFV_OK;
+}
+
+// Grid description and time-control parameters of a configuration, for
+// fields whose rows are `pitch` doubles apart.
+void grid_from_config(const hfv_mh2d_config* cfg, long long pitch, Grid2D& G, RunParams& rp)
+{
     G.nx = cfg->num_cells_x;
     G.ny = cfg->num_cells_y;
     G.g = cfg->num_ghost_cells_side;
     G.tx = G.nx + 2 * G.g;
     G.ty = G.ny + 2 * G.g;
-    G.pitch = ((long long)G.tx + 31) / 32 * 32;
+    G.pitch = pitch;
     G.gamma = cfg->gamma;
     G.gravity = cfg->gravity;
     G.tol = cfg->tol;
@@ -631,7 +634,7 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
     G.nsx = (G.nx + STRIP_COLS - 1) / STRIP_COLS;
     G.strip_rows = cfg->strip_rows > 0 ? cfg->strip_rows : balanced_strip_rows(G.nx, G.ny, resident_step_waves(cfg));
     G.nsy = (G.ny + G.strip_rows - 1) / G.strip_rows;
-    RunParams& rp = h->rp;
+    rp = RunParams();
     rp.cfl = cfg->cfl;
     rp.shrink_factor = cfg->cfl_initial_shrink_factor;
     rp.shrink_steps = cfg->cfl_initial_shrink_num_steps;
@@ -641,6 +644,38 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
     rp.fixed_dt = cfg->fixed_dt;
     rp.dx = cfg->dx;
     rp.dy = cfg->dy;
+}
+
+void fill_status(const Ctrl& c, const RunParams& rp, hfv_run_status* st)
+{
+    st->t = c.t;
+    st->dt = c.dt;
+    st->t_prev = c.t_prev;
+    st->num_steps = c.num_steps;
+    st->err_kind = c.err_kind;
+    st->err_cell = c.err_cell;
+    st->cur = c.cur;
+    st->smax[0] = smax_from_key(c.key[0]);
+    st->smax[1] = smax_from_key(c.key[1]);
+    const bool finished =
+        c.err_kind != ERR_NONE || !(c.t < rp.tf) || (rp.max_steps >= 0 && c.num_steps >= rp.max_steps);
+    st->finished = finished ? 1 : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
+{
+    *out = nullptr;
+    const int rc = check_config(cfg);
+    if (rc != HFV_OK) return rc;
+    hfv_mh2d* h = new (std::nothrow) hfv_mh2d();
+    if (!h) return HFV_ENOMEM;
+    h->cfg = *cfg;
+    Grid2D& G = h->G;
+    grid_from_config(cfg, ((long long)cfg->num_cells_x + 2 * cfg->num_ghost_cells_side + 31) / 32 * 32, G, h->rp);
 
     const size_t field = (size_t)G.pitch * G.ty;
     const size_t bytes = field * 12 * sizeof(double) + 3 * sizeof(Ctrl) + 256;
@@ -819,18 +854,7 @@ int hfv_mh2d_status(hfv_mh2d* h, hfv_run_status* st, void* stream)
     const int slot = (int)(h->launches % 3);
     if (hipMemcpyAsync(&c, h->ctrl + slot, sizeof(Ctrl), hipMemcpyDeviceToHost, s) != hipSuccess) return HFV_EDEVICE;
     if (hipStreamSynchronize(s) != hipSuccess) return HFV_EDEVICE;
-    st->t = c.t;
-    st->dt = c.dt;
-    st->t_prev = c.t_prev;
-    st->num_steps = c.num_steps;
-    st->err_kind = c.err_kind;
-    st->err_cell = c.err_cell;
-    st->cur = c.cur;
-    st->smax[0] = smax_from_key(c.key[0]);
-    st->smax[1] = smax_from_key(c.key[1]);
-    const bool finished = c.err_kind != ERR_NONE || !(c.t < h->rp.tf) ||
-                          (h->rp.max_steps >= 0 && c.num_steps >= h->rp.max_steps);
-    st->finished = finished ? 1 : 0;
+    fill_status(c, h->rp, st);
     return HFV_OK;
 }
 
