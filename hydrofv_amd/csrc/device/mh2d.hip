@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "hfv_ctrl.hpp"
 #include "hfv_gpu_internal.hpp"
@@ -375,6 +376,27 @@ __global__ void k_ctrl_init(Ctrl* ctrl, double t0, long long n0)
     }
 }
 
+// Copy of the 4 fields of every cell (ghosts included) from src to dst; with
+// a controller slot, only when the slot names buffer `when_cur` as current.
+__global__ void k_copy_fields(Grid2D G, Fields4 dst, Fields4 src, const Ctrl* ctrl, int slot, int when_cur)
+{
+    if (ctrl && ctrl[slot].cur != when_cur) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= G.tx) return;
+    const long long k = (long long)blockIdx.y * G.pitch + c;
+#pragma unroll
+    for (int f = 0; f < 4; f++) dst.f[f][k] = src.f[f][k];
+}
+
+// Working memory of hfv_mh2d_run_noalloc on each GPU: the three controller
+// slots in its first 256 bytes, then the second conserved buffer.  Host code
+// takes its address per device with hipGetSymbolAddress (scratch_base) and
+// never uses the name itself.
+constexpr long long NOALLOC_DOUBLES = HFV_NOALLOC_SCRATCH_BYTES / 8;
+constexpr long long NOALLOC_HEAD = 256 / 8;  // doubles before the fields
+static_assert(3 * sizeof(Ctrl) <= 256, "controller slots fit the scratch head");
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
 }  // namespace hfv
 
 // ===========================================================================
@@ -662,6 +684,27 @@ void fill_status(const Ctrl& c, const RunParams& rp, hfv_run_status* st)
     st->finished = finished ? 1 : 0;
 }
 
+// This device's address of `scratch`, looked up on the first call on the
+// device (nullptr if the lookup fails).
+double* scratch_base()
+{
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex mu;
+    static double* base[MAX_DEVICES] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!base[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        base[dev] = static_cast<double*>(p);
+    }
+    return base[dev];
+}
+
 }  // namespace
 
 extern "C" {
@@ -931,6 +974,62 @@ int hfv_mh2d_fields(hfv_mh2d* h, double** U0, double** U1, double** W, long long
         if (W) W[f] = h->W.f[f];
     }
     if (pitch) *pitch = h->G.pitch;
+    return HFV_OK;
+}
+
+int hfv_mh2d_run_noalloc(const hfv_mh2d_config* cfg, double* const W[4], double* const U[4], long long pitch,
+                         long long launches, hfv_run_status* st, void* stream)
+{
+    if (!cfg || !W || !U || launches < 1) return HFV_EINVAL;
+    int rc = check_config(cfg);
+    if (rc != HFV_OK) return rc;
+    // the exact solver's face queues scale with the grid; hooks need the host
+    if (cfg->riemann_solver != HLLC) return HFV_EINVAL;
+    for (int s = 0; s < 4; s++)
+        if (cfg->bc[s] == BC_CUSTOM) return HFV_EINVAL;
+    for (int f = 0; f < 4; f++)
+        if (!W[f] || !U[f]) return HFV_EINVAL;
+    const long long tx = (long long)cfg->num_cells_x + 2 * cfg->num_ghost_cells_side;
+    const long long ty = (long long)cfg->num_cells_y + 2 * cfg->num_ghost_cells_side;
+    if (pitch < tx) return HFV_EINVAL;
+    if (4 * pitch * ty > NOALLOC_DOUBLES - NOALLOC_HEAD) return HFV_ETOOBIG;
+    double* base = scratch_base();
+    if (!base) return HFV_EDEVICE;
+
+    // a run description on the host stack; its buffers are the caller's
+    // arrays and the scratch
+    hfv_mh2d h{};
+    h.cfg = *cfg;
+    grid_from_config(cfg, pitch, h.G, h.rp);
+    const Grid2D& G = h.G;
+    h.ctrl = reinterpret_cast<Ctrl*>(base);
+    for (int f = 0; f < 4; f++) {
+        h.U[0].f[f] = U[f];
+        h.U[1].f[f] = base + NOALLOC_HEAD + pitch * ty * f;
+        h.W.f[f] = W[f];
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 cgrid((unsigned)((G.tx + 255) / 256), (unsigned)G.ty), cblock(256);
+    // both buffers start from the same conserved state (cells a step does
+    // not write stay valid in either)
+    hipLaunchKernelGGL(k_copy_fields, cgrid, cblock, 0, s, G, h.U[1], h.U[0], (const Ctrl*)nullptr, 0, 0);
+    hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, h.ctrl, 0.0, 0LL);
+    hipLaunchKernelGGL(k_smax_from_w, dim3((unsigned)std::min(G.ny, 2048)), dim3(SMAX_THREADS), 0, s, G, h.W,
+                       h.ctrl, 0);
+    launch_step(&h, true, s);  // first step from the caller's primitive state
+    for (long long i = 1; i < launches; i++) launch_step(&h, false, s);
+    const int slot = (int)(launches % 3);
+    hipLaunchKernelGGL(k_copy_fields, cgrid, cblock, 0, s, G, h.U[0], h.U[1], (const Ctrl*)h.ctrl, slot, 1);
+    prim_from_u(G, h.U[0], h.W, s);
+    if (hipGetLastError() != hipSuccess) return HFV_EDEVICE;
+    if (st) {
+        Ctrl c;
+        if (hipMemcpyAsync(&c, h.ctrl + slot, sizeof(Ctrl), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return HFV_EDEVICE;
+        fill_status(c, h.rp, st);
+        st->cur = 0;  // the state is back in the caller's U
+    }
     return HFV_OK;
 }
 

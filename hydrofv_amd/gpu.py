@@ -10,6 +10,8 @@ from . import abi
 dp = C.POINTER(C.c_double)
 
 HFV_OK = 0
+HFV_EINVAL, HFV_ENOMEM, HFV_EDEVICE, HFV_ETOOBIG = -1, -2, -3, -4
+HFV_NOALLOC_SCRATCH_BYTES = 64 << 20
 ERR_NAMES = {0: "none", 1: "dt is zero", 2: "HLLC invalid wave speed", 3: "exact: bracket failure",
              4: "exact: bisection did not converge"}
 
@@ -81,6 +83,8 @@ PROTOTYPES = {
     "hfv_mh2d_status": (C.c_int, [C.c_void_p, _P(RunStatus), C.c_void_p]),
     "hfv_mh2d_download": (C.c_int, [C.c_void_p, _P(dp), _P(dp), C.c_void_p]),
     "hfv_mh2d_fields": (C.c_int, [C.c_void_p, _P(C.c_void_p), _P(C.c_void_p), _P(C.c_void_p), _P(C.c_longlong)]),
+    "hfv_mh2d_run_noalloc": (C.c_int, [_P(MH2DConfig), _P(C.c_void_p), _P(C.c_void_p), C.c_longlong, C.c_longlong,
+                                       _P(RunStatus), C.c_void_p]),
     "hfv_mh2d_set_device_bc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "hfv_register_device_bc_2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "hfv_mh1d_run": (C.c_int, [_P(MH1DConfig), _P(dp), _P(dp), dp, dp, dp, _P(RunStatus), C.c_void_p]),

@@ -21,6 +21,7 @@ extern "C" {
 #define HFV_EINVAL (-1)   /* unsupported or inconsistent configuration */
 #define HFV_ENOMEM (-2)   /* device allocation failed */
 #define HFV_EDEVICE (-3)  /* a HIP runtime call failed */
+#define HFV_ETOOBIG (-4)  /* the grid needs more than the fixed scratch of hfv_mh2d_run_noalloc */
 
 /* Device failure kinds reported by a run (see hfv_run_status.err_kind). */
 #define HFV_ERR_NONE 0
@@ -132,6 +133,34 @@ typedef void (*hfv_device_bc_2d_func)(double *density, double *velocity_x, doubl
 int hfv_mh2d_set_device_bc(hfv_mh2d *h, int side, hfv_device_bc_2d_func fn, void *user);
 /* Device addresses of the two conserved buffers and the primitive buffer. */
 int hfv_mh2d_fields(hfv_mh2d *h, double **U0, double **U1, double **W, long long *pitch);
+
+/* ---- a 2D run that allocates nothing ------------------------------------ */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The run of hfv_mh2d_create + upload + start + advance +
+ * download, with the same kernels and the same results, on the caller's own
+ * device arrays: W[4] (primitive state, ghost cells filled, as the first step
+ * reads it) and U[4] (conserved state), rows `pitch` doubles apart (pitch >=
+ * num_cells_x + 2 g), each num_cells_y + 2 g rows.  It enqueues `launches`
+ * step launches (>= 1; each executes a step unless the run has finished: t >=
+ * tf, max_steps reached, or a failure); when the enqueued work ends, U holds
+ * the final conserved state and W its c2p plus boundary fill.  The library's
+ * working memory (the second conserved buffer and the time-control slots)
+ * lives in a static device array of HFV_NOALLOC_SCRATCH_BYTES on each GPU it
+ * runs on, held until the process ends; a grid with 4 (num_cells_y + 2 g)
+ * pitch doubles beyond HFV_NOALLOC_SCRATCH_BYTES - 256 returns HFV_ETOOBIG.
+ * Only the HLLC solver (riemann_solver 2) and the built-in boundary
+ * conditions 1-4; strip_rows and hllc_form as in hfv_mh2d_create.  Nothing
+ * is allocated, freed or captured: plain kernel launches on `stream` (NULL:
+ * the null stream).  st != NULL synchronises the stream and reports the
+ * run's state; st == NULL synchronises nothing, so the call can be captured.
+ * The first call on a device looks up that device's scratch address
+ * (hipGetSymbolAddress) and must not itself be under capture.  Calls share
+ * their GPU's scratch and keep the last call's contents there: calls that
+ * overlap on one GPU (other streams, other host threads) race. */
+#define HFV_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int hfv_mh2d_run_noalloc(const hfv_mh2d_config *cfg, double *const W[4], double *const U[4], long long pitch,
+                         long long launches, hfv_run_status *st, void *stream);
 
 /* ---- batched ensemble (no reference counterpart) ---------------------- */
 /* Independent 2D problems of uneven size advanced together, one launch per
