@@ -158,6 +158,7 @@ class BoundaryConditionParam(C.Structure):  # boundary.h:35-56
 
 
 _P = C.POINTER
+_DP = C.POINTER(C.c_double)
 _SEVEN = [
     _P(BoundaryConditionParam),
     _P(System),
@@ -195,6 +196,24 @@ PROTOTYPES = {
     "random_choice_1d": (ErrorStatus, _SEVEN),
     "store_snapshot": (ErrorStatus, [_P(BoundaryConditionParam), _P(System), _P(IntegratorParam),
                                      _P(SimulationStatus), _P(StoringParam)]),
+    # per-interface and per-cell functions (slope_limiter.h, riemann_solver*.h, utils.h, source_term.h)
+    "limit_slope": (ErrorStatus, [_DP, _P(IntegratorParam), C.c_double, C.c_double]),
+    "solve_flux_1d": (ErrorStatus, [_P(IntegratorParam), _P(Settings)] + [_DP] * 3 + [C.c_double] * 7),
+    "solve_flux_2d": (ErrorStatus, [_P(IntegratorParam), _P(Settings)] + [_DP] * 4 + [C.c_double] * 9),
+    "riemann_A_L_or_R": (C.c_double, [C.c_double] * 2),
+    "riemann_B_L_or_R": (C.c_double, [C.c_double] * 2),
+    "guess_p": (C.c_double, [C.c_double] * 10),
+    "solve_vacuum_1d": (None, [_DP] * 3 + [C.c_double] * 10),
+    "solve_vacuum_2d": (None, [_DP] * 4 + [C.c_double] * 12),
+    "solve_flux_hllc_1d": (ErrorStatus, [_DP] * 3 + [C.c_double] * 8),
+    "solve_flux_hllc_2d": (ErrorStatus, [_DP] * 4 + [C.c_double] * 10),
+    "solve_exact_1d": (ErrorStatus, [_DP] * 3 + [C.c_double] * 9 + [C.c_int]),
+    "solve_flux_exact_1d": (ErrorStatus, [_DP] * 3 + [C.c_double] * 9 + [C.c_int]),
+    "get_sound_speed": (C.c_double, [C.c_double] * 3),
+    "get_time_step_1d": (C.c_double, [_P(System), C.c_double]),
+    "get_time_step_2d": (C.c_double, [_P(System), C.c_double]),
+    "add_geometry_source_term": (ErrorStatus, [_P(BoundaryConditionParam), _P(System), C.c_double]),
+    "add_gravity_source_term_2d": (ErrorStatus, [_P(BoundaryConditionParam), _P(System), C.c_double]),
 }
 
 

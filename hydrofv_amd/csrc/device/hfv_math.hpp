@@ -728,7 +728,9 @@ HFV_HD __forceinline__ int newton_run(double gamma, Side L, Side R, double tol, 
 }
 
 // After a Newton loop that did not converge: bracketing and bisection.
-HFV_HD __forceinline__ PStar newton_finish(double gamma, Side L, Side R, double tol, const NewtonState& s)
+// *bisections (when given): the bisection iterations taken.
+HFV_HD __forceinline__ PStar newton_finish(double gamma, Side L, Side R, double tol, const NewtonState& s,
+                                           int* bisections = nullptr)
 {
     double p_lo = s.p_lo, f_lo = s.f_lo, p_hi = s.p_hi, f_hi = s.f_hi;
     if (!s.bracketed) {
@@ -757,6 +759,7 @@ HFV_HD __forceinline__ PStar newton_finish(double gamma, Side L, Side R, double 
     for (;;) {
         const double pm = 0.5 * (p_lo + p_hi);
         if (p_hi - p_lo < tol) {
+            if (bisections) *bisections = count;
             return PStar{pm, ERR_NONE};
         }
         const double fm = pressure_fn(gamma, L, R, pm);
@@ -768,16 +771,19 @@ HFV_HD __forceinline__ PStar newton_finish(double gamma, Side L, Side R, double 
             f_lo = fm;
         }
         count++;
-        if (count > 5000) return PStar{0.0, ERR_EXACT_BISECT};
+        if (count > 5000) {
+            if (bisections) *bisections = count;
+            return PStar{0.0, ERR_EXACT_BISECT};
+        }
     }
 }
 
-HFV_HD __forceinline__ PStar solve_p_star(double gamma, Side L, Side R, double tol)
+HFV_HD __forceinline__ PStar solve_p_star(double gamma, Side L, Side R, double tol, int* bisections = nullptr)
 {
     NewtonState s = newton_begin(gamma, L, R, tol);
     double ps;
     if (newton_run(gamma, L, R, tol, s, 100, &ps) > 0) return PStar{ps, ERR_NONE};
-    return newton_finish(gamma, L, R, tol, s);
+    return newton_finish(gamma, L, R, tol, s, bisections);
 }
 
 // exact.c:765-974 (left), 817-1080 (right; note its own operand orders).
@@ -892,9 +898,12 @@ HFV_HD __forceinline__ Sample exact_sample_star(double gamma, Side L, double vl,
 // DIRECT_ONLY: return rc = ERR_PENDING for every problem the shortcut
 // does not answer (the shortcut's answer is the same instructions as in the
 // full solve).
+// *bisections (when given): bisection iterations of the pressure solve (0
+// when it took none).
 template <bool SHORTCUT = true, bool DIRECT_ONLY = false>
 HFV_HD __forceinline__ Sample exact_sample(double gamma, double tol, double rl, double ul, double vl, double pl,
-                                           double rr, double ur, double vr, double pr, double speed)
+                                           double rr, double ur, double vr, double pr, double speed,
+                                           int* bisections = nullptr)
 {
     Sample S;
     S.us = __builtin_nan("");
@@ -934,7 +943,7 @@ HFV_HD __forceinline__ Sample exact_sample(double gamma, double tol, double rl, 
         return S;
     }
     const Side L{rl, ul, pl, al}, R{rr, ur, pr, ar};
-    return exact_sample_star(gamma, L, vl, R, vr, solve_p_star(gamma, L, R, tol), speed);
+    return exact_sample_star(gamma, L, vl, R, vr, solve_p_star(gamma, L, R, tol, bisections), speed);
 }
 
 // The exact solve of a face in two parts, for a solver that sets aside the
@@ -1086,6 +1095,57 @@ HFV_HD __forceinline__ void c2p_1d(double gamma, double vol, double m, double mx
     w[0] = m / vol;
     w[1] = mx / m;
     w[2] = (gamma - 1.0) * (e - 0.5 * m * w[1] * w[1]) / vol;
+}
+
+// ---------------------------------------------------------------------------
+// Source terms (source_term.c).  Rates of the geometric source of one cell
+// (:28-43): pre = -alpha V / r_mid, rho the density, sm and se the specific
+// momentum and energy; o = (mass, momentum, energy) rates.
+HFV_HD __forceinline__ void geo_rate(double pre, double gamma, double rho, double sm, double se, double* o)
+{
+    const double pr = (gamma - 1) * (se - 0.5 * sm * sm / rho);
+    o[0] = pre * sm;
+    o[1] = pre * sm * sm / rho;
+    o[2] = pre * sm * (se + pr) / rho;
+}
+
+// One cell's classical RK4 step of the geometric source (:95-175; alpha 1
+// cylindrical, 2 spherical): u = (m, mx, E) updated in place, rho the
+// cell's density before the step.
+HFV_HD __forceinline__ void geometry_source_rk4(double alpha, double gamma, double vol, double mid, double rho,
+                                                double dt, double* u)
+{
+    const double m = u[0], mx = u[1], e = u[2];
+    const double pre = -alpha * vol / mid;
+    double k1[3], k2[3], k3[3], k4[3];
+    geo_rate(pre, gamma, rho, mx / vol, e / vol, k1);
+    double tm = m + 0.5 * dt * k1[0], tu = mx + 0.5 * dt * k1[1], te = e + 0.5 * dt * k1[2];
+    geo_rate(pre, gamma, tm / vol, tu / vol, te / vol, k2);
+    tm = m + 0.5 * dt * k2[0];
+    tu = mx + 0.5 * dt * k2[1];
+    te = e + 0.5 * dt * k2[2];
+    geo_rate(pre, gamma, tm / vol, tu / vol, te / vol, k3);
+    tm = m + dt * k3[0];
+    tu = mx + dt * k3[1];
+    te = e + dt * k3[2];
+    geo_rate(pre, gamma, tm / vol, tu / vol, te / vol, k4);
+    u[0] += dt * (k1[0] + 2.0 * k2[0] + 2.0 * k3[0] + k4[0]) / 6.0;
+    u[1] += dt * (k1[1] + 2.0 * k2[1] + 2.0 * k3[1] + k4[1]) / 6.0;
+    u[2] += dt * (k1[2] + 2.0 * k2[2] + 2.0 * k3[2] + k4[2]) / 6.0;
+}
+
+// Gravity on one cell (:213-240): the y momentum gains -dt g m, the energy
+// its RK4 integral; m is unchanged.
+HFV_HD __forceinline__ void gravity_source(double dt, double gravity, double m, double* my, double* e)
+{
+    const double pre = -dt * gravity;
+    const double kmy = pre * m;
+    const double k1 = pre * *my;
+    const double k2 = pre * (*my + 0.5 * kmy);
+    const double k3 = k2;
+    const double k4 = pre * (*my + kmy);
+    *my += kmy;
+    *e += (k1 + 2.0 * k2 + 2.0 * k3 + k4) / 6.0;
 }
 
 // CFL reduction key: the maximum of non-negative doubles as an unsigned

@@ -32,6 +32,14 @@
 #define HFV_HD __host__ __device__
 #endif
 
+// Host builds of the reference's own C API (hfv_physics.cpp) define
+// HFV_HOST_LIBM_POW: there every power is the host C library's pow(), the
+// function the reference calls, so those entry points return the
+// reference's bits.  Device code always takes the double-double path.
+#if defined(HFV_HOST_LIBM_POW) && !defined(__HIP_DEVICE_COMPILE__)
+#define HFV_LIBM_POW 1
+#endif
+
 namespace hfv {
 
 struct DD {
@@ -138,6 +146,9 @@ HFV_HD __forceinline__ double exp_dd(DD w, const double* et = pow_exp_tab())
 // The logarithm pow_pos takes of x: log |x| for finite nonzero x, else 0.
 HFV_HD __forceinline__ DD pow_log(double x, const double* lt = pow_log_tab())
 {
+#ifdef HFV_LIBM_POW
+    return DD{0.0, 0.0};  // unused: pow_with_log calls pow()
+#endif
     const double ax = fabs(x);
     const bool regular = (ax > 0.0) & (ax < INFINITY);  // false for 0, inf, NaN
     return log_dd(regular ? ax : 1.0, lt);
@@ -147,6 +158,9 @@ HFV_HD __forceinline__ DD pow_log(double x, const double* lt = pow_log_tab())
 // share its logarithm with the same bits as separate pow_pos calls.
 HFV_HD __forceinline__ double pow_with_log(double x, double y, DD l, const double* et = pow_exp_tab())
 {
+#ifdef HFV_LIBM_POW
+    return pow(x, y);
+#endif
     const double ax = fabs(x);
     const bool regular = (ax > 0.0) & (ax < INFINITY);
     const double wh = y * l.hi;

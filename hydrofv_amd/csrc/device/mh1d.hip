@@ -105,14 +105,6 @@ __device__ __forceinline__ double van_der_corput(long long n)
 }
 
 // source_term.c:28-43
-__device__ __forceinline__ void geo_rate(double pre, double gamma, double rho, double sm, double se, double* o)
-{
-    const double pr = (gamma - 1) * (se - 0.5 * sm * sm / rho);
-    o[0] = pre * sm;
-    o[1] = pre * sm * sm / rho;
-    o[2] = pre * sm * (se + pr) / rho;
-}
-
 // Runs steps until the run finishes or 'budget' steps have been taken (the
 // host relaunches until finished, so no single launch is unbounded).
 // LIM == FIRST_ORDER: godunov_first_order_1d (integrator_godunov_first_order.c:
@@ -331,20 +323,11 @@ __global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays
             // source_term.c:45-175: RK4 per interior cell, then c2p and fill
             const double alpha = (G.coord == 4) ? 1.0 : 2.0;
             for (int k = g + t; k < g + n; k += T1D) {
-                const double vol = A.vol[k];
-                const double m = A.u[0][k], mx = A.u[1][k], e = A.u[2][k];
-                const double pre = -alpha * vol / A.mid[k];
-                double k1[3], k2[3], k3[3], k4[3];
-                geo_rate(pre, gamma, A.w[0][k], mx / vol, e / vol, k1);
-                double tm = m + 0.5 * dt * k1[0], tu = mx + 0.5 * dt * k1[1], te = e + 0.5 * dt * k1[2];
-                geo_rate(pre, gamma, tm / vol, tu / vol, te / vol, k2);
-                tm = m + 0.5 * dt * k2[0]; tu = mx + 0.5 * dt * k2[1]; te = e + 0.5 * dt * k2[2];
-                geo_rate(pre, gamma, tm / vol, tu / vol, te / vol, k3);
-                tm = m + dt * k3[0]; tu = mx + dt * k3[1]; te = e + dt * k3[2];
-                geo_rate(pre, gamma, tm / vol, tu / vol, te / vol, k4);
-                A.u[0][k] += dt * (k1[0] + 2.0 * k2[0] + 2.0 * k3[0] + k4[0]) / 6.0;
-                A.u[1][k] += dt * (k1[1] + 2.0 * k2[1] + 2.0 * k3[1] + k4[1]) / 6.0;
-                A.u[2][k] += dt * (k1[2] + 2.0 * k2[2] + 2.0 * k3[2] + k4[2]) / 6.0;
+                double u[3] = {A.u[0][k], A.u[1][k], A.u[2][k]};
+                geometry_source_rk4(alpha, gamma, A.vol[k], A.mid[k], A.w[0][k], dt, u);
+                A.u[0][k] = u[0];
+                A.u[1][k] = u[1];
+                A.u[2][k] = u[2];
             }
             __syncthreads();
             c2p_all(G, A, t);

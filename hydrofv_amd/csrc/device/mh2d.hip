@@ -193,16 +193,7 @@ __global__ void __launch_bounds__(UPD_THREADS) k_update_deferred(Grid2D G, RunPa
                 x -= D.fy[f][k] * dt * G.area_y;
                 u[f] = x;
             }
-            if (G.gravity != 0.0) {
-                const double pre = -dt * G.gravity;
-                const double kmy = pre * u[0];
-                const double k1 = pre * u[2];
-                const double k2 = pre * (u[2] + 0.5 * kmy);
-                const double k3 = k2;
-                const double k4 = pre * (u[2] + kmy);
-                u[2] += kmy;
-                u[3] += (k1 + 2.0 * k2 + 2.0 * k3 + k4) / 6.0;
-            }
+            if (G.gravity != 0.0) gravity_source(dt, G.gravity, u[0], &u[2], &u[3]);
 #pragma unroll
             for (int f = 0; f < 4; f++) V.f[f][k] = u[f];
             double w[4];

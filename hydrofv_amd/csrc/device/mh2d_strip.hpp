@@ -637,17 +637,8 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
                 x -= fyt[f] * dt * G.area_y;
                 u[f] = x;
             }
-            if (G.gravity != 0.0) {
-                // source_term.c:213-240 on the updated state
-                const double pre = -dt * G.gravity;
-                const double kmy = pre * u[0];
-                const double k1 = pre * u[2];
-                const double k2 = pre * (u[2] + 0.5 * kmy);
-                const double k3 = k2;
-                const double k4 = pre * (u[2] + kmy);
-                u[2] += kmy;
-                u[3] += (k1 + 2.0 * k2 + 2.0 * k3 + k4) / 6.0;
-            }
+            // source_term.c:213-240 on the updated state
+            if (G.gravity != 0.0) gravity_source(dt, G.gravity, u[0], &u[2], &u[3]);
 #pragma unroll
             for (int f = 0; f < 4; f++) V.f[f][k] = u[f];
             // CFL maxima of the new state (utils.c:75-92)

@@ -218,6 +218,51 @@ ErrorStatus set_boundary_condition_cartesian_2d(const BoundaryConditionParam *bo
 ErrorStatus get_slope_limiter_flag(IntegratorParam *integrator_param);   /* slope_limiter.h:26-28 */
 ErrorStatus get_riemann_solver_flag(IntegratorParam *integrator_param);  /* riemann_solver.h:33 */
 
+/* The per-interface and per-cell functions, on the host with the
+ * reference's arithmetic (the integrators run their device forms). */
+ErrorStatus limit_slope(double *limited_slope, const IntegratorParam *integrator_param, const double slope_L,
+                        const double slope_R);                       /* slope_limiter.h:42-47 */
+/* riemann_solver.h:54-85.  solve_flux_2d also takes the exact solver, as the
+ * integrators here do (the reference raises "Only HLLC Riemann solver is
+ * available for cartesian 2D."). */
+ErrorStatus solve_flux_1d(IntegratorParam *integrator_param, Settings *settings, double *flux_mass,
+                          double *flux_momentum, double *flux_energy, const double gamma, const double rho_L,
+                          const double u_L, const double p_L, const double rho_R, const double u_R,
+                          const double p_R);
+ErrorStatus solve_flux_2d(IntegratorParam *integrator_param, Settings *settings, double *flux_mass,
+                          double *flux_momentum_x, double *flux_momentum_y, double *flux_energy, const double gamma,
+                          const double rho_L, const double u_L, const double v_L, const double p_L,
+                          const double rho_R, const double u_R, const double v_R, const double p_R);
+double riemann_A_L_or_R(const double gamma, const double rho_X);      /* riemann_solver.h:95 */
+double riemann_B_L_or_R(const double gamma, const double p_X);        /* riemann_solver.h:105 */
+double guess_p(const double gamma, const double rho_L, const double u_L, const double p_L, const double a_L,
+               const double rho_R, const double u_R, const double p_R, const double a_R,
+               const double tol);                                    /* riemann_solver.h:123-134 */
+void solve_vacuum_1d(double *sol_rho, double *sol_u, double *sol_p, const double gamma, const double rho_L,
+                     const double u_L, const double p_L, const double a_L, const double rho_R, const double u_R,
+                     const double p_R, const double a_R, const double speed); /* riemann_solver.h:153-167 */
+void solve_vacuum_2d(double *sol_rho, double *sol_u, double *sol_v, double *sol_p, const double gamma,
+                     const double rho_L, const double u_L, const double v_L, const double p_L, const double a_L,
+                     const double rho_R, const double u_R, const double v_R, const double p_R, const double a_R,
+                     const double speed);                            /* riemann_solver.h:169-186 */
+ErrorStatus solve_flux_hllc_1d(double *flux_mass, double *flux_momentum, double *flux_energy, const double gamma,
+                               const double rho_L, const double u_L, const double p_L, const double rho_R,
+                               const double u_R, const double p_R,
+                               const double tol);                    /* riemann_solver_hllc.h:32-44 */
+ErrorStatus solve_flux_hllc_2d(double *flux_mass, double *flux_momentum_x, double *flux_momentum_y,
+                               double *flux_energy, const double gamma, const double rho_L, const double u_L,
+                               const double v_L, const double p_L, const double rho_R, const double u_R,
+                               const double v_R, const double p_R,
+                               const double tol);                    /* riemann_solver_hllc.h:46-61 */
+ErrorStatus solve_exact_1d(double *sol_rho, double *sol_u, double *sol_p, const double gamma, const double rho_L,
+                           const double u_L, const double p_L, const double rho_R, const double u_R,
+                           const double p_R, const double tol, const double speed,
+                           const int verbose);                       /* riemann_solver_exact.h:37-51 */
+ErrorStatus solve_flux_exact_1d(double *flux_mass, double *flux_momentum, double *flux_energy, const double gamma,
+                                const double rho_L, const double u_L, const double p_L, const double rho_R,
+                                const double u_R, const double p_R, const double tol, const double speed,
+                                const int verbose);                  /* riemann_solver_exact.h:73-87 */
+
 /* ---- integrator.h:17-176 ----------------------------------------------- */
 #define INTEGRATOR_RANDOM_CHOICE_1D 1
 #define INTEGRATOR_GODUNOV_FIRST_ORDER_1D 2
@@ -283,6 +328,15 @@ ErrorStatus store_snapshot(const BoundaryConditionParam *boundary_condition_para
 
 /* ---- utils.h:25-69 ----------------------------------------------------- */
 double hydro_get_current_time(void);                                 /* utils.h:25 */
+double get_sound_speed(double gamma, double rho, double p);          /* utils.h:32 */
+double get_time_step_1d(const System *system, const double cfl);     /* utils.h:48-51 */
+double get_time_step_2d(const System *system, const double cfl);     /* utils.h:66-69 */
+
+/* ---- source_term.h:21-31 (host; the integrators apply them on the device) */
+ErrorStatus add_geometry_source_term(const BoundaryConditionParam *boundary_condition_param, System *system,
+                                     const double dt);               /* source_term.h:21-25 */
+ErrorStatus add_gravity_source_term_2d(const BoundaryConditionParam *boundary_condition_param, System *system,
+                                       const double dt);             /* source_term.h:27-31 */
 
 /* ---- hydro.h:53-61 ----------------------------------------------------- */
 ErrorStatus launch_simulation(HFV_INTEGRATOR_ARGS);
