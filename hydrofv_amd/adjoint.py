@@ -126,8 +126,9 @@ class Periodic2D:
         return v
 
     def run(self, U0: torch.Tensor, steps: int) -> torch.Tensor:
-        """U after `steps` steps (no autograd)."""
-        a, b = U0.clone(), torch.empty_like(U0)
+        """U after `steps` steps (no autograd).  Ghost entries of the result
+        are defined (both buffers start as U0) but not part of the state."""
+        a, b = U0.clone(), U0.clone()
         for _ in range(steps):
             self.step(a, b)
             a, b = b, a
@@ -140,7 +141,9 @@ class _Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, U0, grid: Periodic2D, steps: int, every: int):
         ckpts = [U0.detach().clone()]
-        a, b = U0.detach().clone(), torch.empty_like(U0)
+        # both buffers start as U0, so the result's ghost entries are defined;
+        # they are not part of the state and carry no gradient
+        a, b = U0.detach().clone(), U0.detach().clone()
         for n in range(steps):
             grid.step(a, b)
             a, b = b, a
@@ -155,6 +158,9 @@ class _Rollout(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_out):
         grid, steps, every = ctx.grid, ctx.steps, ctx.every
+        if steps == 0:  # the identity
+            ctx.ckpts = None
+            return grad_out.clone(), None, None, None
         lam = torch.zeros(grid.shape(), dtype=torch.float64, device=grad_out.device)
         grid.interior(lam).copy_(grid.interior(grad_out))
         other = torch.zeros_like(lam)

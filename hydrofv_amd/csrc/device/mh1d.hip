@@ -490,6 +490,8 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
     if (cfg->steps_start < 0 || !(cfg->t_start >= 0.0) || cfg->t_pause < 0.0) return HFV_EINVAL;
     const bool order1 = cfg->first_order || cfg->random_choice;  // one ghost layer suffices
     if (cfg->num_cells_x < 2 || cfg->num_ghost_cells_side < (order1 ? 1 : 2)) return HFV_EINVAL;
+    // the boundary fill mirrors and wraps interior cells onto the ghosts
+    if (cfg->num_cells_x < cfg->num_ghost_cells_side) return HFV_EINVAL;
     if (!order1 && (cfg->slope_limiter < 1 || cfg->slope_limiter > 3)) return HFV_EINVAL;
     if (!cfg->random_choice && cfg->riemann_solver != HLLC && cfg->riemann_solver != EXACT) return HFV_EINVAL;
     for (int q = 0; q < 2; q++)

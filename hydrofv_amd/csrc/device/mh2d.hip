@@ -246,8 +246,12 @@ __global__ void __launch_bounds__(SMAX_THREADS) k_smax_from_w(Grid2D G, Fields4 
 // fill (x-ghosts on interior rows, y-ghosts on interior columns; the corner
 // blocks keep c2p of their own conserved state), i.e. system.c:763-802
 // followed by boundary.c:513-777.
-__global__ void k_prim_from_u(Grid2D G, Fields4 U, Fields4 W, int fill)
+// With a controller slot: nothing when the run's first step failed in its
+// flux, whose primitive state is the caller's own.
+__global__ void k_prim_from_u(Grid2D G, Fields4 U, Fields4 W, int fill, const Ctrl* ctrl, int slot)
 {
+    if (ctrl && ctrl[slot].err_kind != ERR_NONE && ctrl[slot].err_kind != ERR_DT_ZERO && ctrl[slot].num_steps == 1)
+        return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int q = blockIdx.y;
     if (c >= G.tx) return;
@@ -270,10 +274,11 @@ __global__ void k_prim_from_u(Grid2D G, Fields4 U, Fields4 W, int fill)
 
 // fill = 0: c2p of every cell's own conserved state (system.c:763-802) with
 // no boundary fill
-void prim_from_u(const Grid2D& G, const Fields4& U, const Fields4& W, hipStream_t s, int fill)
+void prim_from_u(const Grid2D& G, const Fields4& U, const Fields4& W, hipStream_t s, int fill,
+                 const Ctrl* ctrl, int slot)
 {
     const dim3 grid((G.tx + 63) / 64, G.ty), block(64);
-    hipLaunchKernelGGL(k_prim_from_u, grid, block, 0, s, G, U, W, fill);
+    hipLaunchKernelGGL(k_prim_from_u, grid, block, 0, s, G, U, W, fill, ctrl, slot);
 }
 
 // c2p of every cell (system.c:763-802) from the conserved buffer that holds
@@ -368,10 +373,17 @@ __global__ void k_ctrl_init(Ctrl* ctrl, double t0, long long n0)
 }
 
 // Copy of the 4 fields of every cell (ghosts included) from src to dst; with
-// a controller slot, only when the slot names buffer `when_cur` as current.
+// a controller slot, only when the slot names buffer `when_cur` as holding
+// the run's result.
 __global__ void k_copy_fields(Grid2D G, Fields4 dst, Fields4 src, const Ctrl* ctrl, int slot, int when_cur)
 {
-    if (ctrl && ctrl[slot].cur != when_cur) return;
+    if (ctrl) {
+        // after a failed flux the result is the state at the start of the
+        // failing step, in the other buffer (as hfv_mh2d_download returns)
+        const Ctrl& c = ctrl[slot];
+        const int result = (c.err_kind != ERR_NONE && c.err_kind != ERR_DT_ZERO) ? c.cur ^ 1 : c.cur;
+        if (result != when_cur) return;
+    }
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= G.tx) return;
     const long long k = (long long)blockIdx.y * G.pitch + c;
@@ -899,12 +911,17 @@ int hfv_mh2d_download(hfv_mh2d* h, double* const W[4], double* const U[4], void*
     int rc = hfv_mh2d_status(h, &st, stream);
     if (rc) return rc;
     const Grid2D& G = h->G;
-    const Fields4 cur = h->U[st.cur];
+    // A step whose flux failed ran to its end on the device, but the
+    // reference stops inside it; the state returned is the one at the start
+    // of that step, which the other buffer still holds.
+    const bool flux_failed = st.err_kind != ERR_NONE && st.err_kind != ERR_DT_ZERO;
+    const Fields4 cur = h->U[flux_failed ? st.cur ^ 1 : st.cur];
     // only when asked for: W may hold a state set by hfv_mh2d_set_primitive
-    if (W && device_bc_mode(h)) {
+    if (W && device_bc_mode(h) && !flux_failed) {
         if (!device_bc_complete(h)) return HFV_EINVAL;
         if (!h->w_pending) fill_primitive(h, s);
-    } else if (W) {
+    } else if (W && !(flux_failed && st.num_steps == 1)) {
+        // (a first step that failed leaves the caller's primitive state in W)
         prim_from_u(G, cur, h->W, s);
     }
     for (int f = 0; f < 4; f++) {
@@ -1011,7 +1028,7 @@ int hfv_mh2d_run_noalloc(const hfv_mh2d_config* cfg, double* const W[4], double*
     for (long long i = 1; i < launches; i++) launch_step(&h, false, s);
     const int slot = (int)(launches % 3);
     hipLaunchKernelGGL(k_copy_fields, cgrid, cblock, 0, s, G, h.U[0], h.U[1], (const Ctrl*)h.ctrl, slot, 1);
-    prim_from_u(G, h.U[0], h.W, s);
+    prim_from_u(G, h.U[0], h.W, s, 1, h.ctrl, slot);
     if (hipGetLastError() != hipSuccess) return HFV_EDEVICE;
     if (st) {
         Ctrl c;

@@ -720,6 +720,7 @@ __device__ __forceinline__ int xcd_remap(int b, int ntiles)
 
 // c2p of every cell of U into W followed by the ghost fill, or with fill = 0
 // without it (mh2d.hip)
-void prim_from_u(const Grid2D& G, const Fields4& U, const Fields4& W, hipStream_t s, int fill = 1);
+void prim_from_u(const Grid2D& G, const Fields4& U, const Fields4& W, hipStream_t s, int fill = 1,
+                 const Ctrl* ctrl = nullptr, int slot = 0);
 
 }  // namespace hfv

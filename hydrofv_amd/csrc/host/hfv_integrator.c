@@ -296,19 +296,19 @@ static ErrorStatus run_2d(BoundaryConditionParam *bc, System *system, Integrator
             if (rc != HFV_OK) break;
             batch = (batch < 48) ? 48 : (batch < 480 ? 2 * batch : 480);
         }
-    } else if (rc == HFV_OK && max_steps >= 0) {
-        /* known step count: enqueue every launch, then check once */
-        if (max_steps > 1) rc = hfv_mh2d_advance(h, max_steps - 1, 1, NULL);
-        if (rc == HFV_OK) rc = hfv_mh2d_status(h, &st, NULL);
     } else if (rc == HFV_OK) {
-        /* run to tf: batches of queued launches between status checks; a
-         * launch after the end only forwards the state */
-        long long batch = 15;
+        /* batches of queued launches between status checks, growing to 480;
+         * a launch after the end only forwards the state.  With a step limit
+         * no more than the remaining max_steps - 1 launches are queued, and
+         * a run that reaches tf or fails first stops at its next check. */
+        long long batch = 15, left = (max_steps >= 0) ? (max_steps > 1 ? max_steps - 1 : 0) : -1;
         for (;;) {
             rc = hfv_mh2d_status(h, &st, NULL);
-            if (rc != HFV_OK || st.finished) break;
-            rc = hfv_mh2d_advance(h, batch, 1, NULL);
+            if (rc != HFV_OK || st.finished || left == 0) break;
+            const long long n = (left > 0 && left < batch) ? left : batch;
+            rc = hfv_mh2d_advance(h, n, 1, NULL);
             if (rc != HFV_OK) break;
+            if (left > 0) left -= n;
             batch = (batch < 48) ? 48 : (batch < 480 ? 2 * batch : 480);
         }
     }

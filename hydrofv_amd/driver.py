@@ -202,6 +202,7 @@ class Simulation:
         if entry.startswith("hfv_"):
             args += [-1 if max_steps is None else int(max_steps), 0.0 if fixed_dt is None else float(fixed_dt)]
         self.last_params = ip
+        self.last_status = status  # also after a failed run
         check(self.lib, fn(*args))
         return RunResult(status.num_steps, status.t, status.dt)
 

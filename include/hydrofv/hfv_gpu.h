@@ -99,7 +99,8 @@ int hfv_mh2d_set_pause(hfv_mh2d *h, double t_pause, long long resume_n);
 int hfv_mh2d_status(hfv_mh2d *h, hfv_run_status *st, void *stream);
 /* Primitive (c2p + ghost fill) and conserved state back to host arrays;
  * either pointer array may be NULL (with W NULL the device's primitive
- * buffer is left as it is). */
+ * buffer is left as it is).  After a failed flux (err_kind 2-4) this is the
+ * state at the start of the failing step. */
 int hfv_mh2d_download(hfv_mh2d *h, double *const W[4], double *const U[4], void *stream);
 /* Host-side boundary conditions (custom hooks, boundary.h:17-33): the
  * primitive state of every cell, c2p of its own conserved state with no

@@ -124,6 +124,31 @@ def test_rollout_gradient_matches_reference():
         assert err <= 1e-10, (k, err)
 
 
+def test_rollout_identity_and_ghost_entries():
+    """Zero steps are the identity, gradient included.  After an odd number
+    of steps the result's ghost entries are defined values (no memory left
+    unwritten), and the gradient reaches the interior only."""
+    nx, ny = 20, 14
+    W = random_W(nx, ny, 8)
+    grid = grid_for(nx, ny, W, cfl=0.6)
+    U0 = conserved(W, grid.volume, grid.g)
+    U0[:, :2, :] = 0.25  # distinct ghost values
+    wt = torch.tensor(np.random.default_rng(4).standard_normal(U0.shape), device="cuda")
+    ghost = torch.ones(U0.shape, dtype=torch.bool, device="cuda")
+    grid.interior(ghost)[...] = False
+    for steps in (0, 3):
+        Ut = torch.tensor(U0, device="cuda", requires_grad=True)
+        UN = adjoint.rollout(Ut, grid, steps, every=2)
+        (UN * wt).sum().backward()
+        if steps == 0:
+            assert torch.equal(UN.detach(), Ut.detach())
+            assert torch.equal(Ut.grad, wt)
+        else:
+            assert bool(torch.isfinite(UN.detach()).all())
+            assert bool((Ut.grad[ghost] == 0).all())
+            assert bool((grid.interior(Ut.grad) != 0).any())
+
+
 def test_gradient_matches_finite_differences_64():
     """Central differences of the device forward along smooth random
     directions: the adjoint workload (smooth Gaussian random field) on a

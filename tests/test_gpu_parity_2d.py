@@ -234,10 +234,43 @@ def test_hllc_invalid_wave_speed_error(gpulib):
     sim.set_primitive(W, interior_only=True)
     sim.apply_boundary()
     sim.primitive_to_conserved()
+    W0, U0 = K.state(sim)
     with pytest.raises(driver.HydroError) as e:
         sim.launch("muscl_hancock_2d", "riemann_solver_hllc", "minmod", 0.5, tf=1.0)
     assert e.value.code == 2
     assert "Invalid wave speed for the HLLC riemann solver" in e.value.text
+    # the reference stops inside the failing first step with W as given; the
+    # device returns the state at the start of that step
+    W1, U1 = K.state(sim)
+    assert np.array_equal(W1.view(np.uint64), W0.view(np.uint64))
+    assert np.array_equal(U1.view(np.uint64), U0.view(np.uint64))
+
+
+def test_flux_failure_returns_the_state_before_the_failing_step(gpulib):
+    """A failure in a later step: a run of 4 steps continued from a 7-step
+    state whose conserved energy is poisoned in one cell fails in its
+    second step (the first reads the caller's clean primitive state); the
+    state returned is the one after its first step, which a 1-step run from
+    the same poisoned state gives, and the status counts that one step."""
+    setup = lambda lib: K.setup_random_2d(lib, 13, 9, 2, K.BC_SETS["periodic"], 17)
+    runs = []
+    for steps in (1, 4):
+        sim = setup(gpulib)
+        sim.launch("muscl_hancock_2d", "riemann_solver_hllc", "minmod", 0.5, tf=np.inf, max_steps=7)
+        sim.system.energy_[5 * 17 + 6] = np.nan  # row 5, column 6: its pressure becomes NaN
+        if steps == 1:
+            res = sim.launch("muscl_hancock_2d", "riemann_solver_hllc", "minmod", 0.5, tf=np.inf, max_steps=1)
+            assert res.num_steps == 1
+        else:
+            with pytest.raises(driver.HydroError) as e:
+                sim.launch("muscl_hancock_2d", "riemann_solver_hllc", "minmod", 0.5, tf=np.inf, max_steps=4)
+            assert e.value.code == 2
+            assert sim.last_status.num_steps == 1
+        runs.append(K.state(sim))
+        sim.close()
+    (Wa, Ua), (Wb, Ub) = runs
+    assert np.array_equal(Ua.view(np.uint64), Ub.view(np.uint64))
+    assert np.array_equal(Wa.view(np.uint64), Wb.view(np.uint64))
 
 
 def test_probe_sqrt_grouped_root_is_the_library_root(gpulib):
@@ -312,3 +345,18 @@ def test_probe_pow_pos_matches_host_build(gpulib):
         np.testing.assert_array_equal(sdev, shost)
     print(f"pow vs glibc over {len(x)} pressure-guess arguments: pow_pos differs in {np.mean(dev != ref):.2e}, "
           f"device library pow in {np.mean(lib_pow != ref):.2e}")
+
+
+def test_step_limit_far_above_the_run_ends_at_tf(gpulib):
+    """A step limit used as a safety cap (1e9) with a finite tf: the run
+    stops at tf after its own step count, as the run to tf does."""
+    import time
+    setup = lambda lib: K.setup_random_2d(lib, 13, 7, 2, K.BC_SETS["transmissive"], 21)
+    a, b = setup(gpulib), setup(gpulib)
+    ra = a.launch("muscl_hancock_2d", "riemann_solver_hllc", "van_leer", 0.8, 0.05)
+    t0 = time.perf_counter()
+    rb = b.launch("muscl_hancock_2d", "riemann_solver_hllc", "van_leer", 0.8, 0.05, max_steps=10 ** 9)
+    assert time.perf_counter() - t0 < 30.0
+    assert (ra.num_steps, ra.t, ra.dt) == (rb.num_steps, rb.t, rb.dt)
+    for x, y in zip(K.state(a), K.state(b)):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
