@@ -276,3 +276,33 @@ def setup_inflow_1d(lib, n, g=2):
     s.apply_boundary()
     s.primitive_to_conserved()
     return s
+
+
+def field_errors(A, B):
+    """Per-field differences of A against B (arrays of fields): the L1
+    relative difference sum|A-B| / sum|B|, the largest difference relative
+    to the field's largest magnitude, and the largest elementwise relative
+    difference |A-B| / |B| over the entries with B != 0 (an entry where B is
+    0 and A is not counts as infinite).  Entries where B is not finite are
+    compared for equality of their bit patterns instead, and a mismatch
+    there counts as infinite too."""
+    out = []
+    for a, b in zip(A, B):
+        a = np.asarray(a, dtype=np.float64)
+        b = np.asarray(b, dtype=np.float64)
+        fin = np.isfinite(b)
+        bad = bool((a[~fin].view(np.uint64) != b[~fin].view(np.uint64)).any())
+        d = np.abs(a[fin] - b[fin])
+        bb = np.abs(b[fin])
+        l1 = d.sum() / max(bb.sum(), 1e-300)
+        mx = d.max() / max(bb.max(), 1e-300) if d.size else 0.0
+        nz = bb > 0
+        el = (d[nz] / bb[nz]).max() if nz.any() else 0.0
+        if bad or bool((d[~nz] > 0).any()):
+            el = np.inf
+        out.append((float(l1), float(mx), float(el)))
+    return out
+
+
+def format_field_errors(errs, names=("m", "mx", "my", "E")):
+    return ", ".join(f"{n}: L1 {e[0]:.1e} max {e[1]:.1e} elem {e[2]:.1e}" for n, e in zip(names, errs))
