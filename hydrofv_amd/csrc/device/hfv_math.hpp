@@ -120,6 +120,49 @@ HFV_HD __forceinline__ void sqrt_rn(double (&x)[K])
     }
 }
 
+// ---------------------------------------------------------------------------
+// Divisions that share a reciprocal.  a / b of doubles compiles for gfx950
+// to: d = v_div_scale(b), n = v_div_scale(a), r = v_rcp(d) refined by two
+// Newton steps r = fma(r, fma(-d, r, 1), r), q = n r, the residual
+// fma(-d, q, n), then v_div_fmas (fma(residual, r, q), rescaled when the
+// scalings set their flag) and v_div_fixup (special operands, overflow,
+// underflow).  For finite a, b with |a|, |b| in [2^-300, 2^300] the scalings
+// return their operands with the flag clear, and the fix-up returns the
+// quotient as it is (none of its cases arises, and a nonzero finite
+// quotient already has the right sign): rcp_core(b) followed by
+// div_core(a, b, r), the same refinement in the same order, gives the bits
+// of a / b there, and one reciprocal serves every division by b.
+// div_core_ok(x): x lies in that range (false for 0, infinities and NaN).
+// Host builds divide.
+HFV_HD __forceinline__ bool div_core_ok(double x)
+{
+    const double ax = fabs(x);
+    return (ax >= 0x1p-300) & (ax <= 0x1p300);
+}
+
+HFV_HD __forceinline__ double rcp_core(double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    r = fma(r, fma(-b, r, 1.0), r);
+    return r;
+#else
+    return b;  // unused: div_core divides
+#endif
+}
+
+HFV_HD __forceinline__ double div_core(double a, double b, double r)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+#else
+    (void)r;
+    return a / b;
+#endif
+}
+
 // utils.c:23-26
 HFV_HD __forceinline__ double sound_speed(double gamma, double rho, double p)
 {
@@ -605,6 +648,60 @@ HFV_HD __forceinline__ bool trrs_settle(double tol, HllcFace& F, double pzl, dou
     return false;
 }
 
+// trrs_settle's decision without the powers, for a two-rarefaction face
+// whose pressures are within 12% of each other.  With z = h/gamma,
+// X = num / (al pl^-z + ar pr^-z) lies below plo^z exactly when
+// num < a_lo + a_hi (plo/phi)^z, where a_lo is the sound speed on the side
+// of the lower pressure plo and a_hi on the side of phi.  (plo/phi)^z =
+// exp(z log(plo/phi)) is evaluated with log(plo/phi) = 2 atanh(s),
+// s = (plo - phi)/(plo + phi) in [-1/16, 0] (plo - phi is exact there), by
+// their Taylor series to s^13 and y^9 (y = z log(plo/phi), |y| < 0.07 for
+// any gamma > 1): truncation below 2^-50 and rounding a few ulps, so the
+// right-hand side is within 2^-47 of its value.  The test with the margin
+// 1 - 2^-39 then implies X < (1 - 2^-40) plo^z in exact arithmetic.  The
+// reference's guess (riemann_solver.c:191-199) computes X from num, which
+// it forms by the same operations, with relative error below 4 ulps, and
+// raises it to 1/z with pow's error below 1 ulp; as (1 - 2^-40)(1 + 4u) < 1,
+// its p* is below plo for every gamma > 1, and with plo >= tol the clamp
+// to tol keeps it there.  q_L = q_R = 1 then, as with ps = plo, and p*
+// enters the flux nowhere else: the face is settled exactly as by
+// trrs_settle, and no power needs to be taken.  F.ps holds num on entry.
+HFV_HD __forceinline__ bool trrs_settle_direct(double gamma, double tol, HllcFace& F)
+{
+    const double pl = F.wl[3], pr = F.wr[3];
+    const bool left_lo = !(pl > pr);
+    const double plo = left_lo ? pl : pr, phi = left_lo ? pr : pl;
+    const double alo = left_lo ? F.al : F.ar, ahi = left_lo ? F.ar : F.al;
+    const double s = (plo - phi) / (plo + phi);
+    const double t = s * s;
+    double q = 1.0 / 13.0;
+    q = fma(q, t, 1.0 / 11.0);
+    q = fma(q, t, 1.0 / 9.0);
+    q = fma(q, t, 1.0 / 7.0);
+    q = fma(q, t, 1.0 / 5.0);
+    q = fma(q, t, 1.0 / 3.0);
+    q = fma(q, t, 1.0);
+    const double y = ((0.5 * (gamma - 1.0)) / gamma) * (2.0 * s * q);
+    double e = 1.0 / 362880.0;
+    e = fma(e, y, 1.0 / 40320.0);
+    e = fma(e, y, 1.0 / 5040.0);
+    e = fma(e, y, 1.0 / 720.0);
+    e = fma(e, y, 1.0 / 120.0);
+    e = fma(e, y, 1.0 / 24.0);
+    e = fma(e, y, 1.0 / 6.0);
+    e = fma(e, y, 0.5);
+    e = fma(e, y, 1.0);
+    const double pz = fma(e, y, 1.0);  // (plo/phi)^z
+    const double rhs = fma(ahi, pz, alo);
+    const double num = F.ps;
+    if ((s >= -0.0625) & (plo >= tol) & (num >= 0.0) & (alo >= 0.0) & (num < (1.0 - 0x1p-39) * rhs)) {
+        F.ps = plo;
+        F.kind = HLLC_READY;
+        return true;
+    }
+    return false;
+}
+
 HFV_HD __forceinline__ int hllc_end(double gamma, const HllcFace& F, double* f)
 {
     int e = hllc_end_regular(gamma, F, f);
@@ -1042,32 +1139,46 @@ HFV_HD __forceinline__ int flux(double gamma, double tol, const double* wl, cons
 // slopes, c = dt*A/V, a = sound speed.  lo = state at the cell's low face,
 // hi = state at its high face.  The y-sweep's density row multiplies the
 // x-velocity slope (977-982, 1002-1007); that form is kept.
-HFV_HD __forceinline__ void predict_x(const double* s, const double* d, double c, double a, double* lo, double* hi)
+// cr: the value of c / s[0], which the predictor takes as the caller has
+// it (the strip step's grouped divisions, divide_by_rho)
+HFV_HD __forceinline__ void predict_x_cr(const double* s, const double* d, double c, double cr, double a, double* lo,
+                                         double* hi)
 {
     const double up = 1.0 + c * s[1];
     const double um = 1.0 - c * s[1];
     lo[0] = s[0] + -0.5 * (up * d[0] + c * s[0] * d[1] + 0.0 + 0.0);
-    lo[1] = s[1] + -0.5 * (0.0 + up * d[1] + 0.0 + (c / s[0]) * d[3]);
+    lo[1] = s[1] + -0.5 * (0.0 + up * d[1] + 0.0 + cr * d[3]);
     lo[2] = s[2] + -0.5 * (0.0 + 0.0 + up * d[2] + 0.0);
     lo[3] = s[3] + -0.5 * (0.0 + c * s[0] * a * a * d[1] + 0.0 + up * d[3]);
     hi[0] = s[0] + 0.5 * (um * d[0] - c * s[0] * d[1] + 0.0 + 0.0);
-    hi[1] = s[1] + 0.5 * (0.0 + um * d[1] + 0.0 - (c / s[0]) * d[3]);
+    hi[1] = s[1] + 0.5 * (0.0 + um * d[1] + 0.0 - cr * d[3]);
     hi[2] = s[2] + 0.5 * (0.0 + 0.0 + um * d[2] + 0.0);
     hi[3] = s[3] + 0.5 * (0.0 - c * s[0] * a * a * d[1] + 0.0 + um * d[3]);
 }
 
-HFV_HD __forceinline__ void predict_y(const double* s, const double* d, double c, double a, double* lo, double* hi)
+HFV_HD __forceinline__ void predict_x(const double* s, const double* d, double c, double a, double* lo, double* hi)
+{
+    predict_x_cr(s, d, c, c / s[0], a, lo, hi);
+}
+
+HFV_HD __forceinline__ void predict_y_cr(const double* s, const double* d, double c, double cr, double a, double* lo,
+                                         double* hi)
 {
     const double vp = 1.0 + c * s[2];
     const double vm = 1.0 - c * s[2];
     lo[0] = s[0] + -0.5 * (vp * d[0] + c * s[0] * d[1] + 0.0 + 0.0);
     lo[1] = s[1] + -0.5 * (0.0 + vp * d[1] + 0.0 + 0.0);
-    lo[2] = s[2] + -0.5 * (0.0 + 0.0 + vp * d[2] + (c / s[0]) * d[3]);
+    lo[2] = s[2] + -0.5 * (0.0 + 0.0 + vp * d[2] + cr * d[3]);
     lo[3] = s[3] + -0.5 * (0.0 + 0.0 + c * s[0] * a * a * d[2] + vp * d[3]);
     hi[0] = s[0] + 0.5 * (vm * d[0] - c * s[0] * d[1] + 0.0 + 0.0);
     hi[1] = s[1] + 0.5 * (0.0 + vm * d[1] + 0.0 + 0.0);
-    hi[2] = s[2] + 0.5 * (0.0 + 0.0 + vm * d[2] - (c / s[0]) * d[3]);
+    hi[2] = s[2] + 0.5 * (0.0 + 0.0 + vm * d[2] - cr * d[3]);
     hi[3] = s[3] + 0.5 * (0.0 + 0.0 - c * s[0] * a * a * d[2] + vm * d[3]);
+}
+
+HFV_HD __forceinline__ void predict_y(const double* s, const double* d, double c, double a, double* lo, double* hi)
+{
+    predict_y_cr(s, d, c, c / s[0], a, lo, hi);
 }
 
 HFV_HD __forceinline__ void predict_1d(const double* s, const double* d, double c, double a, double* lo, double* hi)
@@ -1088,6 +1199,27 @@ HFV_HD __forceinline__ void c2p_2d(double gamma, double vol, double m, double mx
     w[1] = mx / m;
     w[2] = my / m;
     w[3] = (gamma - 1.0) * (e - 0.5 * m * (w[1] * w[1] + w[2] * w[2])) / vol;
+}
+
+// c2p_2d with the divisions by m sharing one reciprocal and those by the
+// (uniform) cell volume sharing rvol = rcp_core(vol), vol_ok =
+// div_core_ok(vol), when every lane's operands allow; the bits of c2p_2d.
+HFV_HD __forceinline__ void c2p_2d_rn(double gamma, double vol, double rvol, bool vol_ok, double m, double mx,
+                                      double my, double e, double* w)
+{
+    if (wave_all(vol_ok & div_core_ok(m) & div_core_ok(mx) & div_core_ok(my))) {
+        const double rm = rcp_core(m);
+        w[0] = div_core(m, vol, rvol);
+        w[1] = div_core(mx, m, rm);
+        w[2] = div_core(my, m, rm);
+    } else {
+        w[0] = m / vol;
+        w[1] = mx / m;
+        w[2] = my / m;
+    }
+    const double num = (gamma - 1.0) * (e - 0.5 * m * (w[1] * w[1] + w[2] * w[2]));
+    if (wave_all(vol_ok & div_core_ok(num))) w[3] = div_core(num, vol, rvol);
+    else w[3] = num / vol;
 }
 
 HFV_HD __forceinline__ void c2p_1d(double gamma, double vol, double m, double mx, double e, double* w)

@@ -457,6 +457,21 @@ __global__ void __launch_bounds__(LANES) k_probe_sqrt(int impl, long long n, con
     if (i < n) out[i] = r;
 }
 
+__global__ void __launch_bounds__(LANES) k_probe_div(int impl, long long n, const double* a, const double* b,
+                                                     double* out)
+{
+    const long long i = (long long)blockIdx.x * LANES + threadIdx.x;
+    const double x = a[(i < n) ? i : 0], y = b[(i < n) ? i : 0];
+    double r;
+    if (impl == 1) {
+        if (wave_all(div_core_ok(x) & div_core_ok(y))) r = div_core(x, y, rcp_core(y));
+        else r = x / y;
+    } else {
+        r = (impl == 2) ? div_core(x, y, rcp_core(y)) : x / y;
+    }
+    if (i < n) out[i] = r;
+}
+
 __global__ void k_probe_limit(int lim, long long n, const double* a, const double* b, double* out)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -659,6 +674,22 @@ int hfv_probe_sqrt(int impl, long long n, const double* x, double* out)
     if (!dx.alloc((size_t)n) || !dout.alloc((size_t)n)) return HFV_ENOMEM;
     if (hipMemcpy(dx.p, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return HFV_EDEVICE;
     hipLaunchKernelGGL(k_probe_sqrt, dim3((unsigned)((n + LANES - 1) / LANES)), dim3(LANES), 0, 0, impl, n, dx.p,
+                       dout.p);
+    if (hipGetLastError() != hipSuccess) return HFV_EDEVICE;
+    if (hipMemcpy(out, dout.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return HFV_EDEVICE;
+    return HFV_OK;
+}
+
+int hfv_probe_div(int impl, long long n, const double* a, const double* b, double* out)
+{
+    if (n < 0 || impl < 0 || impl > 2) return HFV_EINVAL;
+    if (n == 0) return HFV_OK;
+    DevBuf<double> da, db, dout;
+    if (!da.alloc((size_t)n) || !db.alloc((size_t)n) || !dout.alloc((size_t)n)) return HFV_ENOMEM;
+    if (hipMemcpy(da.p, a, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(db.p, b, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return HFV_EDEVICE;
+    hipLaunchKernelGGL(k_probe_div, dim3((unsigned)((n + LANES - 1) / LANES)), dim3(LANES), 0, 0, impl, n, da.p, db.p,
                        dout.p);
     if (hipGetLastError() != hipSuccess) return HFV_EDEVICE;
     if (hipMemcpy(out, dout.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return HFV_EDEVICE;

@@ -13,7 +13,7 @@
 namespace hfv {
 
 template <int LIM, int SOLVER, bool WMODE, bool BRANCHY>
-__global__ void __launch_bounds__(LANES) k_mh2d_step(Grid2D G, RunParams rp, Fields4 Ua, Fields4 Ub, Fields4 W,
+__global__ void __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(2))) k_mh2d_step(Grid2D G, RunParams rp, Fields4 Ua, Fields4 Ub, Fields4 W,
                                                      Ctrl* ctrl, int slot)
 {
     const Ctrl* in = ctrl + slot;

@@ -106,14 +106,15 @@ __device__ __forceinline__ void load_raw(const Grid2D& G, const Fields4& U, cons
 
 // Primitive state from loaded values: c2p (system.c:763-802) and the
 // boundary flips.
+// rvol, vol_ok: rcp_core(G.volume), div_core_ok(G.volume) (c2p_2d_rn).
 template <bool WMODE>
-__device__ __forceinline__ void raw_to_prim(const Grid2D& G, const RawCell& rc, double* w)
+__device__ __forceinline__ void raw_to_prim(const Grid2D& G, const RawCell& rc, double* w, double rvol, bool vol_ok)
 {
     if constexpr (WMODE) {
 #pragma unroll
         for (int f = 0; f < 4; f++) w[f] = rc.v[f];
     } else {
-        c2p_2d(G.gamma, G.volume, rc.v[0], rc.v[1], rc.v[2], rc.v[3], w);
+        c2p_2d_rn(G.gamma, G.volume, rvol, vol_ok, rc.v[0], rc.v[1], rc.v[2], rc.v[3], w);
         if (rc.fu) w[1] = -w[1];
         if (rc.fv) w[2] = -w[2];
     }
@@ -122,26 +123,41 @@ __device__ __forceinline__ void raw_to_prim(const Grid2D& G, const RawCell& rc, 
 // Primitive state of cell (row q, column c) as the reference's stencil sees it.
 template <bool WMODE>
 __device__ __forceinline__ void load_prim(const Grid2D& G, const Fields4& U, const Fields4& W, int q, int c,
-                                          double* w, RawCell* keep = nullptr)
+                                          double* w, double rvol, bool vol_ok)
 {
     RawCell rc;
     load_raw<WMODE>(G, U, W, q, c, rc);
-    raw_to_prim<WMODE>(G, rc, w);
-    if (keep) *keep = rc;
+    raw_to_prim<WMODE>(G, rc, w, rvol, vol_ok);
 }
 
-// y-predictor of the cell whose own state is wc, with wb below and wt above.
-// Returns the cell's sound speed, which its x-predictor uses a row later.
+// y-predictor of the cell whose own state is wc, with wb below and wt above
+// (c = dt A_y / V).  Returns the cell's sound speed and *cxr = cx / rho
+// (cx = dt A_x / V), which its x-predictor uses a row later.  The three
+// divisions by the cell's density (the sound speed's gamma p / rho, c / rho
+// and cx / rho) share one reciprocal when every lane's operands allow
+// (div_core); the bits of sound_speed and predict_y / predict_x.
 template <int LIM>
 __device__ __forceinline__ double ypredict(const double* wb, const double* wc, const double* wt, double c,
-                                           double gamma, double* lo, double* hi)
+                                           double cx, double gamma, double* lo, double* hi, double* cxr)
 {
     double d[4];
 #pragma unroll
     for (int f = 0; f < 4; f++) d[f] = limit<LIM>(wc[f] - wb[f], wt[f] - wc[f]);
-    const double a = sound_speed_rn(gamma, wc[0], wc[3]);
-    predict_y(wc, d, c, a, lo, hi);
-    return a;
+    const double rho = wc[0], gp = gamma * wc[3];
+    double a2[1], cyr;
+    if (wave_all(div_core_ok(rho) & div_core_ok(gp) & div_core_ok(c) & div_core_ok(cx))) {
+        const double rr = rcp_core(rho);
+        a2[0] = div_core(gp, rho, rr);
+        cyr = div_core(c, rho, rr);
+        *cxr = div_core(cx, rho, rr);
+    } else {
+        a2[0] = gp / rho;
+        cyr = c / rho;
+        *cxr = cx / rho;
+    }
+    sqrt_rn<1>(a2);
+    predict_y_cr(wc, d, c, cyr, a2[0], lo, hi);
+    return a2[0];
 }
 
 // y-sweep flux between the top face of the lower cell (hi) and the bottom
@@ -176,8 +192,9 @@ __device__ __forceinline__ void stage_pow_tables(double* pow_lds, int lane)
 }
 
 // The powers of the two-rarefaction pressure guesses pending on two faces per
-// lane (riemann_solver.c:191-199), evaluated 64 at a time over the
-// wavefront: the lanes' pending bases are packed into LDS, each lane raises
+// lane (riemann_solver.c:191-199) that trrs_settle_direct cannot settle
+// (about 1 face in 1000 on a Kelvin-Helmholtz run), evaluated 64 at a time
+// over the wavefront: the lanes' pending bases are packed into LDS, each lane raises
 // one packed value per pass (pow_pos, hfv_pow.hpp), and the results are
 // read back.  A row of a
 // strip has about 30 such faces in 128, so the first two powers take one or
@@ -188,7 +205,12 @@ __device__ __forceinline__ void stage_pow_tables(double* pow_lds, int lane)
 __device__ __forceinline__ void trrs_powers(double gamma, double tol, HllcFace& A, HllcFace& B, double* q, int lane,
                                             const double* pow_lds)
 {
-    const bool ta = A.kind == HLLC_TRRS, tb = B.kind == HLLC_TRRS;
+    // most faces are settled without powers (trrs_settle_direct); the
+    // passes below take the others
+    bool ta = A.kind == HLLC_TRRS, tb = B.kind == HLLC_TRRS;
+    if (__ballot(ta | tb) == 0ull) return;
+    if (ta) ta = !trrs_settle_direct(gamma, tol, A);
+    if (tb) tb = !trrs_settle_direct(gamma, tol, B);
     const unsigned long long ma = __ballot(ta), mb = __ballot(tb);
     if ((ma | mb) == 0ull) return;
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -421,6 +443,8 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     static_assert(!DEFER || SOLVER == EXACT, "only exact-solver faces are queued");
     const double cx = dt * G.area_x / G.volume;  // dt*A/V, integrator_muscl_hancock.c:636
     const double cy = dt * G.area_y / G.volume;  // :921
+    const double rvol = rcp_core(G.volume);      // the c2p divisions by the cell volume (c2p_2d_rn)
+    const bool vol_ok = div_core_ok(G.volume);
 
     const int g = G.g;
     const int x0 = g + sx * STRIP_COLS;        // first output column of the strip
@@ -449,18 +473,20 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
 
     // ---- prologue: rows y0-2 .. y0+1 (first order: y0-1 .. y0) ----------
     double wa[4], wb[4], wc[4];
-    double a_row = 0.0, a_next = 0.0;  // sound speeds of rows r and r+1 (MUSCL)
+    double a_row = 0.0, a_next = 0.0;      // sound speeds of rows r and r+1 (MUSCL)
+    double cxr_row = 0.0, cxr_next = 0.0;  // cx / rho of rows r and r+1
     double yt[4], fyb[4];
-    // The conserved state loaded for rows r and r+1 (a row's c2p reads it two
-    // iterations before the row's update, which then needs no second load:
-    // inside the grid the boundary mapping is the identity), and the next
-    // row's, loaded an iteration ahead.  WMODE loads primitive states, so its
-    // updates load U.
-    RawCell raw_a, raw_b, next;
+    // The next row's stored values, loaded an iteration ahead.  A row's update
+    // loads the row's conserved state again, at the top of its iteration and
+    // before that prefetch: the vector-memory counter completes in order, so
+    // the update then waits for its own loads only (they hit in cache), and
+    // no lane holds the conserved states of the rows between their c2p and
+    // their update.
+    RawCell next;
     if constexpr (LIM == FIRST_ORDER) {
         double wm1[4];
-        load_prim<WMODE>(G, U, W, y0 - 1, col, wm1);
-        load_prim<WMODE>(G, U, W, y0, col, wa, &raw_a);
+        load_prim<WMODE>(G, U, W, y0 - 1, col, wm1, rvol, vol_ok);
+        load_prim<WMODE>(G, U, W, y0, col, wa, rvol, vol_ok);
 #pragma unroll
         for (int f = 0; f < 4; f++) {
             wb[f] = wa[f];  // unused
@@ -475,13 +501,13 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         }
     } else {
         double wm2[4], wm1[4];
-        load_prim<WMODE>(G, U, W, y0 - 2, col, wm2);
-        load_prim<WMODE>(G, U, W, y0 - 1, col, wm1);
-        load_prim<WMODE>(G, U, W, y0, col, wa, &raw_a);
-        load_prim<WMODE>(G, U, W, y0 + 1, col, wb, &raw_b);
-        double lo[4], hi_m1[4], lo0[4];
-        ypredict<LIM>(wm2, wm1, wa, cy, gamma, lo, hi_m1);         // row y0-1: top face state
-        a_row = ypredict<LIM>(wm1, wa, wb, cy, gamma, lo0, yt);    // row y0: bottom and top face states
+        load_prim<WMODE>(G, U, W, y0 - 2, col, wm2, rvol, vol_ok);
+        load_prim<WMODE>(G, U, W, y0 - 1, col, wm1, rvol, vol_ok);
+        load_prim<WMODE>(G, U, W, y0, col, wa, rvol, vol_ok);
+        load_prim<WMODE>(G, U, W, y0 + 1, col, wb, rvol, vol_ok);
+        double lo[4], hi_m1[4], lo0[4], unused;
+        ypredict<LIM>(wm2, wm1, wa, cy, cx, gamma, lo, hi_m1, &unused);         // row y0-1: top face state
+        a_row = ypredict<LIM>(wm1, wa, wb, cy, cx, gamma, lo0, yt, &cxr_row);   // row y0: bottom and top face states
         if constexpr (DEFER) {
             hyb = out_col && direct_face<true>(gamma, tol, hi_m1, lo0, fyb);
             queue_face<true>(D, hyb && y0 == g, (long long)(y0 - 1) * G.pitch + col, hi_m1, lo0, lane);
@@ -510,8 +536,13 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         // follows below); x: the face states of row r, with the lane's
         // neighbours taken by shuffles
         double ylo[4], yhi[4], xlo[4], xhi[4];
-        const RawCell raw_c = next;  // row r + AHEAD
-        raw_to_prim<WMODE>(G, raw_c, wc);
+        raw_to_prim<WMODE>(G, next, wc, rvol, vol_ok);  // row r + AHEAD
+        const long long k = (long long)r * G.pitch + col;
+        double uk[4];  // this row's conserved state, for its update
+        if (out_col) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) uk[f] = U.f[f][k];
+        }
         if (r + 1 < y1) load_raw<WMODE>(G, U, W, r + 1 + AHEAD, col, next);
         if constexpr (LIM == FIRST_ORDER) {
 #pragma unroll
@@ -520,7 +551,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
                 xlo[f] = xhi[f] = wa[f];
             }
         } else {
-            a_next = ypredict<LIM>(wa, wb, wc, cy, gamma, ylo, yhi);
+            a_next = ypredict<LIM>(wa, wb, wc, cy, cx, gamma, ylo, yhi, &cxr_next);
             double wl[4], wr[4];
 #pragma unroll
             for (int f = 0; f < 4; f++) {
@@ -530,7 +561,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             double d[4];
 #pragma unroll
             for (int f = 0; f < 4; f++) d[f] = limit<LIM>(wa[f] - wl[f], wr[f] - wa[f]);
-            predict_x(wa, d, cx, a_row, xlo, xhi);
+            predict_x_cr(wa, d, cx, cxr_row, a_row, xlo, xhi);
         }
         double hl[4];
 #pragma unroll
@@ -601,7 +632,6 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
 #pragma unroll
         for (int f = 0; f < 4; f++) fxr[f] = shfl_from_right(fxl[f]);
 
-        const long long k = (long long)r * G.pitch + col;
         bool deferred = false;
         if constexpr (DEFER) {
             const bool hxr = __shfl_down((int)hx, 1, LANES) != 0;
@@ -630,7 +660,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             double u[4];
 #pragma unroll
             for (int f = 0; f < 4; f++) {
-                double x = WMODE ? U.f[f][k] : raw_a.v[f];
+                double x = uk[f];
                 x += fxl[f] * dt * G.area_x;
                 x -= fxr[f] * dt * G.area_x;
                 x += fyb[f] * dt * G.area_y;
@@ -643,7 +673,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             for (int f = 0; f < 4; f++) V.f[f][k] = u[f];
             // CFL maxima of the new state (utils.c:75-92)
             double w[4];
-            c2p_2d(gamma, G.volume, u[0], u[1], u[2], u[3], w);
+            c2p_2d_rn(gamma, G.volume, rvol, vol_ok, u[0], u[1], u[2], u[3], w);
             const double a = sound_speed_rn(gamma, w[0], w[3]);
             const double sx_ = a + fabs(w[1]);
             const double sy_ = a + fabs(w[2]);
@@ -664,9 +694,8 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             yt[f] = yhi[f];
             fyb[f] = fyt[f];
         }
-        raw_a = (LIM == FIRST_ORDER) ? raw_c : raw_b;
-        raw_b = raw_c;
         a_row = a_next;
+        cxr_row = cxr_next;
         hyb = hy;
     }
     if constexpr (DEFER) {

@@ -93,6 +93,7 @@ PROTOTYPES = {
     "hfv_probe_limit": (C.c_int, [C.c_int, C.c_longlong, dp, dp, dp]),
     "hfv_probe_pow": (C.c_int, [C.c_int, C.c_longlong, dp, dp, dp]),
     "hfv_probe_sqrt": (C.c_int, [C.c_int, C.c_longlong, dp, dp]),
+    "hfv_probe_div": (C.c_int, [C.c_int, C.c_longlong, dp, dp, dp]),
     "hfv_device_error_string": (C.c_char_p, []),
     "hfv_muscl_hancock_1d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
     "hfv_muscl_hancock_2d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
@@ -163,6 +164,16 @@ def probe_sqrt(lib, impl: int, x: np.ndarray):
     x = np.ascontiguousarray(x, dtype=np.float64)
     out = np.zeros_like(x)
     check(lib, lib.hfv_probe_sqrt(impl, x.size, _ptr(x), _ptr(out)), "hfv_probe_sqrt")
+    return out
+
+
+def probe_div(lib, impl: int, a: np.ndarray, b: np.ndarray):
+    """a / b on the device: impl 0 the division, 1 the kernels' grouped form,
+    2 its unscaled core (operands of magnitude in [2^-300, 2^300] only)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.zeros_like(a)
+    check(lib, lib.hfv_probe_div(impl, a.size, _ptr(a), _ptr(b), _ptr(out)), "hfv_probe_div")
     return out
 
 

@@ -267,6 +267,13 @@ int hfv_probe_pow(int impl, long long n, const double *x, const double *y, doubl
  * The kernels' roots of sound speeds and HLLC wave-speed factors (utils.c:23-26,
  * riemann_solver_hllc.c:30-40) must equal impl 0 bit for bit. */
 int hfv_probe_sqrt(int impl, long long n, const double *x, double *out);
+/* out[n] = a[n] / b[n] as the kernels take it: impl 0 the device's division, impl 1 the
+ * kernels' grouped form (the division's refinement alone, rcp_core / div_core, when every
+ * operand of the wavefront has magnitude in [2^-300, 2^300]; the division otherwise),
+ * impl 2 that core alone (for operands in range).  The kernels' divisions of primitive
+ * conversions and predictors (system.c:792-799, integrator_muscl_hancock.c:692-740,
+ * utils.c:23-26) must equal impl 0 bit for bit. */
+int hfv_probe_div(int impl, long long n, const double *a, const double *b, double *out);
 /* out[n] = limit_slope(a[n], b[n]) for limiter 1/2/3 (slope_limiter.c:93-115). */
 int hfv_probe_limit(int limiter, long long n, const double *a, const double *b, double *out);
 

@@ -98,15 +98,27 @@ void hc_flux_hllc(int nf, double gamma, double tol, long long n, const double* w
     }
 }
 
-// The 2D kernels' split form: hllc_begin, the two-rarefaction powers taken
-// as the wavefront takes them (with trrs_settle), hllc_end.
-void hc_flux_hllc_split(double gamma, double tol, long long n, const double* wl, const double* wr, double* f,
-                        int* err)
+// settled[n] = 1 where trrs_settle_direct settles the face's two-rarefaction
+// guess without powers, 0 where it does not, -1 for faces of another kind
+void hc_trrs_settle_direct(double gamma, double tol, long long n, const double* wl, const double* wr, int* settled)
 {
     for (long long i = 0; i < n; i++) {
         HllcFace F;
         hllc_begin(gamma, tol, wl + 4 * i, wr + 4 * i, F);
-        if (F.kind == HLLC_TRRS) {
+        settled[i] = (F.kind == HLLC_TRRS) ? (trrs_settle_direct(gamma, tol, F) ? 1 : 0) : -1;
+    }
+}
+
+// The 2D kernels' split form: hllc_begin, the two-rarefaction powers taken
+// as the wavefront takes them (trrs_settle_direct when settle_direct, then
+// the powers with trrs_settle), hllc_end.
+void hc_flux_hllc_split(double gamma, double tol, long long n, const double* wl, const double* wr, double* f,
+                        int* err, int settle_direct)
+{
+    for (long long i = 0; i < n; i++) {
+        HllcFace F;
+        hllc_begin(gamma, tol, wl + 4 * i, wr + 4 * i, F);
+        if (F.kind == HLLC_TRRS && !(settle_direct && trrs_settle_direct(gamma, tol, F))) {
             const double h = 0.5 * (gamma - 1.0);
             const double pl = pow_pos(F.wl[3], h / gamma), pr = pow_pos(F.wr[3], h / gamma);
             double x;

@@ -27,7 +27,8 @@ def hc():
         pytest.fail(f"{HOSTCHECK_SO} is not built (make -C tests/native)")
     L = C.CDLL(HOSTCHECK_SO)
     L.hc_flux_hllc.argtypes = [C.c_int, C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp, _ip]
-    L.hc_flux_hllc_split.argtypes = [C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp, _ip]
+    L.hc_flux_hllc_split.argtypes = [C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp, _ip, C.c_int]
+    L.hc_trrs_settle_direct.argtypes = [C.c_double, C.c_double, C.c_longlong, _dp, _dp, _ip]
     L.hc_flux_exact.argtypes = [C.c_int, C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp, _ip]
     L.hc_hllc_vjp.argtypes = [C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp, _dp, _dp, _ip]
     L.hc_predict.argtypes = [C.c_int, C.c_longlong, _dp, _dp, C.c_double, C.c_double, _dp, _dp]
@@ -35,7 +36,7 @@ def hc():
     L.hc_limit.argtypes = [C.c_int, C.c_longlong, _dp, _dp, _dp, _dp, _dp]
     L.hc_c2p_vjp.argtypes = [C.c_double, C.c_double, C.c_longlong, _dp, _dp, _dp]
     L.hc_pow_pos.argtypes = [C.c_longlong, _dp, _dp, _dp]
-    for f in ("hc_pow_pos", "hc_flux_hllc", "hc_flux_hllc_split", "hc_flux_exact", "hc_hllc_vjp", "hc_predict",
+    for f in ("hc_pow_pos", "hc_flux_hllc", "hc_flux_hllc_split", "hc_trrs_settle_direct", "hc_flux_exact", "hc_hllc_vjp", "hc_predict",
               "hc_predict_vjp", "hc_limit", "hc_c2p_vjp"):
         getattr(L, f).restype = None
     return L
@@ -118,7 +119,8 @@ def test_host_hllc_equals_oracle(hc, nf):
     assert np.all(err[~ok] != 0)
 
 
-def test_split_hllc_equals_plain(hc):
+@pytest.mark.parametrize("settle_direct", [0, 1])
+def test_split_hllc_equals_plain(hc, settle_direct):
     """The kernels' split HLLC (begin, powers, end) gives the bits of the
     plain flux_hllc<4>: the kernels use both for the same face depending on
     the strip layout."""
@@ -127,7 +129,7 @@ def test_split_hllc_equals_plain(hc):
     f1, f2 = np.empty((n, 4)), np.empty((n, 4))
     e1, e2 = np.empty(n, np.int32), np.empty(n, np.int32)
     hc.hc_flux_hllc(4, GAMMA, TOL, n, wl, wr, f1, e1)
-    hc.hc_flux_hllc_split(GAMMA, TOL, n, wl, wr, f2, e2)
+    hc.hc_flux_hllc_split(GAMMA, TOL, n, wl, wr, f2, e2, settle_direct)
     assert np.array_equal(e1, e2)
     ok = e1 == 0  # on a failure the flux is left unwritten
     assert np.array_equal(f1[ok], f2[ok], equal_nan=True)
@@ -146,7 +148,7 @@ def test_split_hllc_settles_exactly(hc):
     f1, f2 = np.empty((n, 4)), np.empty((n, 4))
     e1, e2 = np.empty(n, np.int32), np.empty(n, np.int32)
     hc.hc_flux_hllc(4, GAMMA, TOL, n, wl, wr, f1, e1)
-    hc.hc_flux_hllc_split(GAMMA, TOL, n, wl, wr, f2, e2)
+    hc.hc_flux_hllc_split(GAMMA, TOL, n, wl, wr, f2, e2, 1)
     assert np.array_equal(e1, e2)
     ok = e1 == 0
     # split and plain forms: the same bits (NaN: vacuum generation into 0/0)
@@ -442,3 +444,75 @@ def test_exact_solver_capped_and_resumed_is_bitwise(hc):
             if cap == 16:
                 assert 0 < paused.sum() < m
         assert (rc0 != 0).sum() >= 50  # the NaN faces fail (bisection)
+
+
+def _expansions(n, seed, gamma, spread):
+    """Two-rarefaction faces with pressures within `spread` of each other."""
+    r = np.random.default_rng(seed)
+    wl, wr = rng_states(n, seed + 1), rng_states(n, seed + 2)
+    wl[:, 3] = r.uniform(0.2, 3.0, n)
+    wr[:, 3] = wl[:, 3] * (1.0 + r.uniform(-spread, spread, n))
+    wl[:, 1] = -r.uniform(0.0, 0.3, n)
+    wr[:, 1] = r.uniform(0.0, 0.3, n)
+    return wl, wr
+
+
+@pytest.mark.parametrize("gamma", [1.4, 5.0 / 3.0, 1.05, 3.0])
+def test_trrs_settle_direct_is_exact(hc, gamma):
+    """Two-rarefaction faces settled without powers (trrs_settle_direct):
+    the split form with it equals the plain flux_hllc<4> bit for bit, and on
+    every face it settles the flux equals the oracle's (glibc pow) bit for
+    bit, since there the reference's guess lies below both pressures; it
+    settles nearly every expansion with close pressures."""
+    n = 30000
+    wl, wr = _expansions(n, 40, gamma, 0.15)
+    st = np.empty(n, np.int32)
+    hc.hc_trrs_settle_direct(gamma, TOL, n, wl, wr, st)
+    f1, f2 = np.empty((n, 4)), np.empty((n, 4))
+    e1, e2 = np.empty(n, np.int32), np.empty(n, np.int32)
+    hc.hc_flux_hllc(4, gamma, TOL, n, wl, wr, f1, e1)
+    hc.hc_flux_hllc_split(gamma, TOL, n, wl, wr, f2, e2, 1)
+    assert np.array_equal(e1, e2) and np.array_equal(f1, f2, equal_nan=True)
+    fo, eo = O.flux_batch(2, "riemann_solver_hllc", gamma, TOL, wl, wr)
+    s = st == 1
+    assert np.array_equal(f2[s].view(np.uint64), fo[s].view(np.uint64))
+    close = (st >= 0) & (np.minimum(wl[:, 3], wr[:, 3]) >= 0.9 * np.maximum(wl[:, 3], wr[:, 3]))
+    assert close.sum() > 5000 and s[close].mean() > 0.95, (close.sum(), s[close].mean())
+
+
+def test_trrs_settle_direct_at_its_margin(hc):
+    """Faces built so that the exact two-rarefaction base X lies within
+    +-1e-9 (relative) of plo^z, i.e. p* within a few 1e-9 of the lower
+    pressure on either side: the direct test settles none closer than
+    2^-41 and some beyond 2^-38, and every face it settles has the oracle's
+    flux bit for bit."""
+    r = np.random.default_rng(77)
+    n = 20000
+    g = np.longdouble(GAMMA)
+    wl, wr = _expansions(n, 50, GAMMA, 0.1)
+    al = np.sqrt(GAMMA * wl[:, 3] / wl[:, 0])
+    ar = np.sqrt(GAMMA * wr[:, 3] / wr[:, 0])
+    h = 0.5 * (GAMMA - 1.0)
+    z = (0.5 * (g - 1)) / g
+    pl, pr = wl[:, 3].astype(np.longdouble), wr[:, 3].astype(np.longdouble)
+    D = al.astype(np.longdouble) / pl ** z + ar.astype(np.longdouble) / pr ** z
+    plo_z = np.minimum(pl, pr) ** z
+    delta = r.choice([-1.0, 1.0], n) * 10.0 ** r.uniform(-16.0, -9.0, n)
+    num = (1 - delta) * plo_z * D
+    du = ((al + ar).astype(np.longdouble) - num) / np.longdouble(h)
+    wr[:, 1] = wl[:, 1] + du.astype(np.float64)
+    # the realised margin, from the numerator as the solver forms it
+    num_c = (al + ar - h * (wr[:, 1] - wl[:, 1])).astype(np.longdouble)
+    real = 1 - num_c / (plo_z * D)
+    st = np.empty(n, np.int32)
+    hc.hc_trrs_settle_direct(GAMMA, TOL, n, wl, wr, st)
+    tr = st >= 0
+    assert tr.sum() > 0.3 * n  # the others take the PVRS guess (p* near plo puts ppv there too)
+    s = st == 1
+    assert not (s & (real < 2.0 ** -41)).any()
+    assert (s & (real > 2.0 ** -38)).sum() > 1000
+    f2 = np.empty((n, 4))
+    e2 = np.empty(n, np.int32)
+    hc.hc_flux_hllc_split(GAMMA, TOL, n, wl, wr, f2, e2, 1)
+    fo, eo = O.flux_batch(2, "riemann_solver_hllc", GAMMA, TOL, wl, wr)
+    assert np.array_equal(f2[s].view(np.uint64), fo[s].view(np.uint64))

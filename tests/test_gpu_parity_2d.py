@@ -306,6 +306,44 @@ def test_probe_sqrt_grouped_root_is_the_library_root(gpulib):
         np.testing.assert_array_equal(core.view(np.int64), np.sqrt(x).view(np.int64))
 
 
+def test_probe_division_core_is_the_library_division(gpulib):
+    """The kernels' divisions (rcp_core / div_core: the device division's
+    refinement without its scalings and fix-up when every operand of a
+    wavefront has magnitude in [2^-300, 2^300]) give the bits of the
+    device division and of the correctly rounded quotient, for wavefronts
+    entirely in range and for wavefronts holding an operand outside it
+    (0, -0, denormals, the range's neighbours, huge and tiny values,
+    infinities, NaN), which take the division itself."""
+    r = np.random.default_rng(13)
+    n = 64 * 4096
+    def in_range(k):
+        lo = np.float64(2.0 ** -300).view(np.int64)
+        hi = np.float64(2.0 ** 300).view(np.int64)
+        x = r.integers(lo, hi, size=k, endpoint=True).view(np.float64)
+        return x * r.choice([-1.0, 1.0], k)
+    a, b = in_range(n), in_range(n)
+    a[:6] = [2.0 ** -300, 2.0 ** 300, -2.0 ** 300, 1.0, 3.0, 1e-5]
+    b[:6] = [2.0 ** 300, 2.0 ** -300, 2.0 ** -300, 3.0, 1.0, 7.0]
+    # physical ranges: densities, pressures, momenta, dt A / V
+    pa = np.exp(r.uniform(np.log(1e-12), np.log(1e12), n)) * r.choice([-1.0, 1.0], n)
+    pb = np.exp(r.uniform(np.log(1e-8), np.log(1e8), n))
+    special = np.array([0.0, -0.0, 5e-324, 2.0 ** -1022, np.nextafter(2.0 ** -300, 0.0), 2.0 ** 301, 2.0 ** 1000,
+                        np.inf, -np.inf, np.nan, 1e-310, -2.0 ** -400])
+    ma, mb = r.uniform(0.1, 10.0, 64 * len(special)), r.uniform(0.1, 10.0, 64 * len(special))
+    ma[::64] = special
+    mb[32::64] = special
+    for x, y in ((a, b), (pa, pb), (ma, mb), (special, special[::-1].copy())):
+        lib = gpu.probe_div(gpulib, 0, x, y)
+        grp = gpu.probe_div(gpulib, 1, x, y)
+        with np.errstate(all="ignore"):
+            ref = x / y
+        np.testing.assert_array_equal(grp.view(np.int64), lib.view(np.int64))
+        np.testing.assert_array_equal(lib, ref)  # NaN positions compare equal
+    for x, y in ((a, b), (pa, pb)):
+        core = gpu.probe_div(gpulib, 2, x, y)
+        np.testing.assert_array_equal(core.view(np.int64), (x / y).view(np.int64))
+
+
 def test_probe_pow_pos_matches_host_build(gpulib):
     """The kernels' pow (pow_pos) gives the same bits on the
     device as its host build (tests/native), which test_adjoint_math checks
