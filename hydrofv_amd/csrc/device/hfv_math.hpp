@@ -721,36 +721,117 @@ HFV_HD __forceinline__ double wave_f(double gamma, double rho, double p, double 
     return 2.0 * a * (pow_pos((ps / p), (gamma - 1.0) / (2.0 * gamma)) - 1.0) / (gamma - 1.0);
 }
 
-// wave_f and its derivative (exact.c:539-565) at one pressure, for the
-// Newton step.  The derivative's shock branch keeps the (B - p*) of
-// exact.c:553 as written.  The shock branch's square root serves both
-// (ps + B and B + ps are the same sum), and so does the rarefaction
-// branch's log(ps / p) (pow_log); each result has the bits of evaluating
-// the reference's two functions separately.
-HFV_HD __forceinline__ void wave_f_df(double gamma, double rho, double p, double a, double ps, double* f, double* df)
-{
-    if (ps > p) {
-        const double A = coeff_A(gamma, rho);
-        const double B = coeff_B(gamma, p);
-        const double s = sqrt(A / (ps + B));
-        *f = (ps - p) * s;
-        *df = (1.0 - 0.5 * (ps - p) / (B - ps)) * s;
-        return;
-    }
-    const double x = ps / p;
-    const DD l = pow_log(x);
-    *f = 2.0 * a * (pow_with_log(x, (gamma - 1.0) / (2.0 * gamma), l) - 1.0) / (gamma - 1.0);
-    *df = pow_with_log(x, -0.5 * (gamma + 1.0) / gamma, l) / (p * a);
-}
-
 // The side states are (rho, u, p) in the exact solver (3-component order).
 struct Side {
     double r, u, p, a;
 };
 
-HFV_HD __forceinline__ double pressure_fn(double gamma, Side L, Side R, double ps)
+// One side of the pressure function with its coefficients that do not
+// depend on p*: A and B (exact.c:516-519, 543-546), 2a and p a
+// (exact.c:527, 561): the reference's expressions, evaluated once per face
+// instead of once per iteration, so the same bits.
+struct WaveSide {
+    double p, A, B, a2, pa;
+};
+
+HFV_HD __forceinline__ WaveSide wave_side(double gamma, const Side& S)
 {
-    return wave_f(gamma, L.r, L.p, L.a, ps) + wave_f(gamma, R.r, R.p, R.a, ps) + (R.u - L.u);
+    return WaveSide{S.p, coeff_A(gamma, S.r), coeff_B(gamma, S.p), 2.0 * S.a, S.p * S.a};
+}
+
+// The pressure function f(p*) = f_L(p*) + f_R(p*) + (u_R - u_L)
+// (exact.c:567-585) and its derivative, set up once per face for the
+// iteration.  The sides are held with the lower pressure first: IEEE
+// addition commutes, so f_L + f_R and f_R + f_L are the same number, and in
+// this order every lane whose iterate lies between the two pressures takes
+// the shock branch on s0 and the rarefaction branch on s1, whichever of L
+// and R each is, so that a wavefront's lanes diverge less.
+// lt, et: the power tables pow_pos reads (the global ones, or copies).
+struct PressureFn {
+    WaveSide s0, s1;
+    double du;   // u_R - u_L
+    double zf;   // (gamma - 1) / (2 gamma): f's rarefaction exponent
+    double zd;   // -0.5 (gamma + 1) / gamma: f''s
+    double gm1;  // gamma - 1
+    const double* lt;
+    const double* et;
+};
+
+HFV_HD __forceinline__ PressureFn pressure_fn_setup(double gamma, const Side& L, const Side& R,
+                                                    const double* lt = pow_log_tab(),
+                                                    const double* et = pow_exp_tab())
+{
+    const bool swap = R.p < L.p;
+    PressureFn F;
+    F.s0 = wave_side(gamma, swap ? R : L);
+    F.s1 = wave_side(gamma, swap ? L : R);
+    F.du = R.u - L.u;
+    F.zf = (gamma - 1.0) / (2.0 * gamma);
+    F.zd = -0.5 * (gamma + 1.0) / gamma;
+    F.gm1 = gamma - 1.0;
+    F.lt = lt;
+    F.et = et;
+    return F;
+}
+
+// wave_f of one side (exact.c:512-537), by branch: shock (p* > p) or
+// rarefaction
+HFV_HD __forceinline__ double shock_f(const WaveSide& S, double ps)
+{
+    return (ps - S.p) * sqrt(S.A / (ps + S.B));
+}
+
+HFV_HD __forceinline__ double rarefaction_f(const PressureFn& F, const WaveSide& S, double ps)
+{
+    return S.a2 * (pow_pos(ps / S.p, F.zf, F.lt, F.et) - 1.0) / F.gm1;
+}
+
+// wave_f and its derivative (exact.c:539-565) at one pressure, for the
+// Newton step, by branch.  The derivative's shock branch keeps the (B - p*)
+// of exact.c:553 as written.  The shock branch's square root serves both
+// (ps + B and B + ps are the same sum), and so does the rarefaction
+// branch's log(ps / p) (pow_log); each result has the bits of evaluating
+// the reference's two functions separately.
+HFV_HD __forceinline__ void shock_f_df(const WaveSide& S, double ps, double* f, double* df)
+{
+    const double s = sqrt(S.A / (ps + S.B));
+    *f = (ps - S.p) * s;
+    *df = (1.0 - 0.5 * (ps - S.p) / (S.B - ps)) * s;
+}
+
+HFV_HD __forceinline__ void rarefaction_f_df(const PressureFn& F, const WaveSide& S, double ps, double* f, double* df)
+{
+    const double x = ps / S.p;
+    const DD l = pow_log(x, F.lt);
+    *f = S.a2 * (pow_with_log(x, F.zf, l, F.et) - 1.0) / F.gm1;
+    *df = pow_with_log(x, F.zd, l, F.et) / S.pa;
+}
+
+// f(p*) and f'(p*).  The branch each side takes is selected from values
+// computed as follows: the lower-pressure side's shock value and the
+// higher-pressure side's rarefaction value unconditionally (the pair a p*
+// between the two pressures needs, in one stretch of code whose two chains
+// overlap), and the other two under a branch, taken where some lane needs
+// them.  The selected values are the reference's expressions, so the bits
+// are the reference's.
+HFV_HD __forceinline__ double pressure_fn(const PressureFn& F, double ps)
+{
+    double f0 = shock_f(F.s0, ps);
+    double f1 = rarefaction_f(F, F.s1, ps);
+    if (!(ps > F.s0.p)) f0 = rarefaction_f(F, F.s0, ps);
+    if (ps > F.s1.p) f1 = shock_f(F.s1, ps);
+    return f0 + f1 + F.du;
+}
+
+HFV_HD __forceinline__ void pressure_fn_df(const PressureFn& F, double ps, double* f, double* df)
+{
+    double f0, d0, f1, d1;
+    shock_f_df(F.s0, ps, &f0, &d0);
+    rarefaction_f_df(F, F.s1, ps, &f1, &d1);
+    if (!(ps > F.s0.p)) rarefaction_f_df(F, F.s0, ps, &f0, &d0);
+    if (ps > F.s1.p) shock_f_df(F.s1, ps, &f1, &d1);
+    *f = f0 + f1 + F.du;
+    *df = d0 + d1;
 }
 
 struct PStar {
@@ -771,17 +852,13 @@ struct NewtonState {
     int bracketed;
 };
 
-HFV_HD __forceinline__ NewtonState newton_begin(double gamma, Side L, Side R, double tol)
+HFV_HD __forceinline__ NewtonState newton_begin(double gamma, Side L, Side R, double tol, const PressureFn& F)
 {
     const double pg = guess_p(gamma, L.r, L.u, L.p, L.a, R.r, R.u, R.p, R.a, tol);
-    double fl, dfl, fr, dfr;
-    wave_f_df(gamma, L.r, L.p, L.a, pg, &fl, &dfl);
-    wave_f_df(gamma, R.r, R.p, R.a, pg, &fr, &dfr);
     NewtonState s;
     s.p0 = pg;
     s.p_hi = pg;
-    s.f_hi = fl + fr + (R.u - L.u);
-    s.df0 = dfl + dfr;
+    pressure_fn_df(F, pg, &s.f_hi, &s.df0);
     s.p_lo = 0.0;
     s.f_lo = 0.0;
     s.it = 0;
@@ -793,8 +870,7 @@ HFV_HD __forceinline__ NewtonState newton_begin(double gamma, Side L, Side R, do
 // *ps when the iterate converged, -1 when the loop has ended without
 // converging (a negative iterate, or 100 iterations), 0 when it stopped at
 // it_end.
-HFV_HD __forceinline__ int newton_run(double gamma, Side L, Side R, double tol, NewtonState& s, int it_end,
-                                      double* ps)
+HFV_HD __forceinline__ int newton_run(const PressureFn& F, double tol, NewtonState& s, int it_end, double* ps)
 {
     for (; s.it < it_end; s.it++) {
         double f, df;
@@ -802,11 +878,7 @@ HFV_HD __forceinline__ int newton_run(double gamma, Side L, Side R, double tol, 
             f = s.f_hi;
             df = s.df0;
         } else {
-            double fl, dfl, fr, dfr;
-            wave_f_df(gamma, L.r, L.p, L.a, s.p0, &fl, &dfl);
-            wave_f_df(gamma, R.r, R.p, R.a, s.p0, &fr, &dfr);
-            f = fl + fr + (R.u - L.u);
-            df = dfl + dfr;
+            pressure_fn_df(F, s.p0, &f, &df);
         }
         if (!s.bracketed && f * s.f_hi < 0.0) {
             s.bracketed = 1;
@@ -825,22 +897,26 @@ HFV_HD __forceinline__ int newton_run(double gamma, Side L, Side R, double tol, 
 }
 
 // After a Newton loop that did not converge: bracketing and bisection.
+// The scan's f at the upper end of an interval is f at the lower end of the
+// next (the reference evaluates it twice, exact.c:684-685; one evaluation
+// gives the same bits).
 // *bisections (when given): the bisection iterations taken.
-HFV_HD __forceinline__ PStar newton_finish(double gamma, Side L, Side R, double tol, const NewtonState& s,
+HFV_HD __forceinline__ PStar newton_finish(const PressureFn& F, double tol, const NewtonState& s,
                                            int* bisections = nullptr)
 {
     double p_lo = s.p_lo, f_lo = s.f_lo, p_hi = s.p_hi, f_hi = s.f_hi;
     if (!s.bracketed) {
         p_lo = 0.0;
-        f_lo = pressure_fn(gamma, L, R, p_lo);
+        f_lo = pressure_fn(F, p_lo);
         if (f_lo * f_hi >= 0.0) {
             p_hi *= 10.0;
             const double dp = (p_hi - p_lo) / 100;
+            double fb = pressure_fn(F, p_lo + 0 * dp);
             for (int k = 0; k < 100; k++) {
                 const double a = p_lo + k * dp;
                 const double b = p_lo + (k + 1) * dp;
-                const double fa = pressure_fn(gamma, L, R, a);
-                const double fb = pressure_fn(gamma, L, R, b);
+                const double fa = fb;
+                fb = pressure_fn(F, b);
                 if (fa * fb < 0.0) {
                     p_lo = a;
                     f_lo = fa;
@@ -859,7 +935,7 @@ HFV_HD __forceinline__ PStar newton_finish(double gamma, Side L, Side R, double 
             if (bisections) *bisections = count;
             return PStar{pm, ERR_NONE};
         }
-        const double fm = pressure_fn(gamma, L, R, pm);
+        const double fm = pressure_fn(F, pm);
         if (fm * f_lo < 0.0) {
             p_hi = pm;
             f_hi = fm;
@@ -877,10 +953,11 @@ HFV_HD __forceinline__ PStar newton_finish(double gamma, Side L, Side R, double 
 
 HFV_HD __forceinline__ PStar solve_p_star(double gamma, Side L, Side R, double tol, int* bisections = nullptr)
 {
-    NewtonState s = newton_begin(gamma, L, R, tol);
+    const PressureFn F = pressure_fn_setup(gamma, L, R);
+    NewtonState s = newton_begin(gamma, L, R, tol, F);
     double ps;
-    if (newton_run(gamma, L, R, tol, s, 100, &ps) > 0) return PStar{ps, ERR_NONE};
-    return newton_finish(gamma, L, R, tol, s, bisections);
+    if (newton_run(F, tol, s, 100, &ps) > 0) return PStar{ps, ERR_NONE};
+    return newton_finish(F, tol, s, bisections);
 }
 
 // exact.c:765-974 (left), 817-1080 (right; note its own operand orders).
@@ -1066,29 +1143,32 @@ HFV_HD __forceinline__ Sample exact_sample_capped(double gamma, double tol, doub
         return S;
     }
     const Side L{rl, ul, pl, al}, R{rr, ur, pr, ar};
-    *st = newton_begin(gamma, L, R, tol);
+    const PressureFn F = pressure_fn_setup(gamma, L, R);
+    *st = newton_begin(gamma, L, R, tol, F);
     double ps;
-    const int r = newton_run(gamma, L, R, tol, *st, newton_cap, &ps);
+    const int r = newton_run(F, tol, *st, newton_cap, &ps);
     if (r == 0) {
         Sample S;
         S.rc = ERR_PAUSED;
         return S;
     }
-    return exact_sample_star(gamma, L, vl, R, vr, (r > 0) ? PStar{ps, ERR_NONE} : newton_finish(gamma, L, R, tol, *st),
+    return exact_sample_star(gamma, L, vl, R, vr, (r > 0) ? PStar{ps, ERR_NONE} : newton_finish(F, tol, *st),
                              speed);
 }
 
+// lt, et: the power tables the iteration reads (copies in LDS, say)
 HFV_HD __forceinline__ Sample exact_sample_resume(double gamma, double tol, double rl, double ul, double vl,
                                                   double pl, double rr, double ur, double vr, double pr,
-                                                  double speed, NewtonState st)
+                                                  double speed, NewtonState st, const double* lt = pow_log_tab(),
+                                                  const double* et = pow_exp_tab())
 {
     const double al = (rl > 0.0) ? sound_speed(gamma, rl, pl) : 0.0;
     const double ar = (rr > 0.0) ? sound_speed(gamma, rr, pr) : 0.0;
     const Side L{rl, ul, pl, al}, R{rr, ur, pr, ar};
+    const PressureFn F = pressure_fn_setup(gamma, L, R, lt, et);
     double ps;
-    const int r = newton_run(gamma, L, R, tol, st, 100, &ps);
-    return exact_sample_star(gamma, L, vl, R, vr, (r > 0) ? PStar{ps, ERR_NONE} : newton_finish(gamma, L, R, tol, st),
-                             speed);
+    const int r = newton_run(F, tol, st, 100, &ps);
+    return exact_sample_star(gamma, L, vl, R, vr, (r > 0) ? PStar{ps, ERR_NONE} : newton_finish(F, tol, st), speed);
 }
 
 // The flux of the state sampled at x/t = 0 (exact.c:471-510; NF = 4: the

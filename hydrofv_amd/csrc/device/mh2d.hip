@@ -34,8 +34,17 @@ __global__ void __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(2)))
 }
 
 // The exact-solver step with queued faces (Deferred, mh2d_strip.hpp): the
-// strip pass, then k_solve_faces, then k_update_deferred; k_defer_reset
-// clears the queues before the strip pass.
+// strip pass, then k_solve_faces, k_solve_slow and k_update_deferred.  The
+// queue counters come in three sets, one per controller slot: a step uses
+// its slot's set, and its strip pass clears the next slot's, which no
+// launch of this step reads (k_defer_reset clears a set that no step has
+// cleared: the first step of a run description).
+__device__ __forceinline__ Deferred defer_slot(Deferred D, int slot)
+{
+    D.count += 3 * slot;
+    return D;
+}
+
 template <int LIM, bool WMODE>
 __global__ void __launch_bounds__(LANES) k_mh2d_step_defer(Grid2D G, RunParams rp, Fields4 Ua, Fields4 Ub,
                                                            Fields4 W, Ctrl* ctrl, int slot, Deferred D)
@@ -46,22 +55,29 @@ __global__ void __launch_bounds__(LANES) k_mh2d_step_defer(Grid2D G, RunParams r
     int dt_zero;
     const StepPlan plan = plan_step(in, rp, &dt_zero);
     const int lane = threadIdx.x;
-    if (blockIdx.x == 0 && lane == 0) publish(in, out, clear, plan, dt_zero);
+    if (blockIdx.x == 0 && lane == 0) {
+        publish(in, out, clear, plan, dt_zero);
+        unsigned long long* next = D.count + 3 * ((slot + 1) % 3);
+        next[0] = 0ull;
+        next[1] = 0ull;
+        next[2] = 0ull;
+    }
     if (!plan.run) return;
     const Fields4 U = plan.cur ? Ub : Ua;
     const Fields4 V = plan.cur ? Ua : Ub;
     const int tile = xcd_remap(blockIdx.x, G.nsx * G.nsy);
-    const StripResult r =
-        strip_step<LIM, EXACT, WMODE, false, true>(G, plan.dt, U, V, W, tile % G.nsx, tile / G.nsx, lane, D);
+    const StripResult r = strip_step<LIM, EXACT, WMODE, false, true>(G, plan.dt, U, V, W, tile % G.nsx,
+                                                                     tile / G.nsx, lane, defer_slot(D, slot));
     strip_finish(r, out, lane);
 }
 
-__global__ void k_defer_reset(Deferred D)
+__global__ void k_defer_reset(Deferred D, int slot)
 {
     if (threadIdx.x == 0) {
-        D.count[0] = 0ull;
-        D.count[1] = 0ull;
-        D.count[2] = 0ull;
+        unsigned long long* c = D.count + 3 * slot;
+        c[0] = 0ull;
+        c[1] = 0ull;
+        c[2] = 0ull;
     }
 }
 
@@ -100,8 +116,9 @@ constexpr int NEWTON_CAP = 8;
 // One queued face per lane: the exact solve (riemann_solver_exact.c) of
 // exact_sample_capped.  The queue length was written by the strip pass,
 // which ran before this launch on the same stream.
-__global__ void __launch_bounds__(LANES) k_solve_faces(Grid2D G, Deferred D, Ctrl* ctrl, int slot)
+__global__ void __launch_bounds__(LANES) k_solve_faces(Grid2D G, Deferred D0, Ctrl* ctrl, int slot)
 {
+    const Deferred D = defer_slot(D0, slot);
     const long long n = (long long)min(D.count[0], (unsigned long long)D.face_cap);
     Ctrl* out = ctrl + (slot + 1) % 3;
     const int lane = threadIdx.x;
@@ -135,8 +152,9 @@ __global__ void __launch_bounds__(LANES) k_solve_faces(Grid2D G, Deferred D, Ctr
 }
 
 // The set-aside faces, resumed from their saved iteration state.
-__global__ void __launch_bounds__(LANES) k_solve_slow(Grid2D G, Deferred D, Ctrl* ctrl, int slot)
+__global__ void __launch_bounds__(LANES) k_solve_slow(Grid2D G, Deferred D0, Ctrl* ctrl, int slot)
 {
+    const Deferred D = defer_slot(D0, slot);
     const long long n = (long long)min(D.count[2], (unsigned long long)D.slow_cap);
     Ctrl* out = ctrl + (slot + 1) % 3;
     for (long long j = (long long)blockIdx.x * LANES + threadIdx.x; j < n; j += (long long)gridDim.x * LANES) {
@@ -167,11 +185,12 @@ __global__ void __launch_bounds__(LANES) k_solve_slow(Grid2D G, Deferred D, Ctrl
 constexpr int UPD_THREADS = 256;
 
 __global__ void __launch_bounds__(UPD_THREADS) k_update_deferred(Grid2D G, RunParams rp, Fields4 Ua, Fields4 Ub,
-                                                                 Deferred D, Ctrl* ctrl, int slot)
+                                                                 Deferred D0, Ctrl* ctrl, int slot)
 {
     int dt_zero;
     const StepPlan plan = plan_step(ctrl + slot, rp, &dt_zero);
     if (!plan.run) return;
+    const Deferred D = defer_slot(D0, slot);
     const Fields4 U = plan.cur ? Ub : Ua;
     const Fields4 V = plan.cur ? Ua : Ub;
     const double dt = plan.dt, gamma = G.gamma;
@@ -427,6 +446,7 @@ struct hfv_mh2d {
     long long w_filled;    // launch count at which the primitive buffer was last filled (device hooks)
     double* stage;         // one contiguous field for host copies (allocated at first use)
     Deferred D;            // exact solver: queued faces and deferred cells
+    long long defer_clear; // exact solver: the launch whose queue counters are known to be clear
     double* defer_slab;
 };
 
@@ -483,7 +503,8 @@ void launch_step_t(hfv_mh2d* h, bool wmode, hipStream_t s)
     h->launches++;
 }
 
-// exact solver: queue reset, strip pass, queued faces, deferred cells
+// exact solver: strip pass, queued faces, deferred cells (and the first
+// step's queue reset)
 template <int LIM>
 void launch_exact(hfv_mh2d* h, bool wmode, hipStream_t s)
 {
@@ -492,7 +513,7 @@ void launch_exact(hfv_mh2d* h, bool wmode, hipStream_t s)
     const long long cells = (long long)G.tx * G.ty;
     const unsigned solve_blocks = (unsigned)std::min<long long>(4096, (2 * cells + LANES - 1) / LANES);
     const unsigned update_blocks = (unsigned)std::min<long long>(1024, (cells + UPD_THREADS - 1) / UPD_THREADS);
-    hipLaunchKernelGGL(k_defer_reset, dim3(1), dim3(64), 0, s, h->D);
+    if (h->defer_clear != h->launches) hipLaunchKernelGGL(k_defer_reset, dim3(1), dim3(64), 0, s, h->D, slot);
     const dim3 grid(G.nsx * G.nsy), block(LANES);
     if (wmode) {
         hipLaunchKernelGGL((k_mh2d_step_defer<LIM, true>), grid, block, 0, s, G, h->rp, h->U[0], h->U[1], h->W,
@@ -507,6 +528,7 @@ void launch_exact(hfv_mh2d* h, bool wmode, hipStream_t s)
     hipLaunchKernelGGL(k_update_deferred, dim3(update_blocks), dim3(UPD_THREADS), 0, s, G, h->rp, h->U[0], h->U[1],
                        h->D, h->ctrl, slot);
     h->launches++;
+    h->defer_clear = h->launches;  // the strip pass cleared the next slot's counters
 }
 
 template <int LIM>
@@ -744,7 +766,7 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
         // a face that cannot be set aside is finished in the first pass
         D.slow_cap = std::max<long long>(D.face_cap / 8, 64);
         const size_t dbytes =
-            (field * 8 + (size_t)D.face_cap * 9 + (size_t)D.cell_cap + (size_t)D.slow_cap * 7 + 4) * sizeof(double);
+            (field * 8 + (size_t)D.face_cap * 9 + (size_t)D.cell_cap + (size_t)D.slow_cap * 7 + 9) * sizeof(double);
         if (hipMalloc(&h->defer_slab, dbytes) != hipSuccess) {
             (void)hipFree(h->slab);
             delete h;
@@ -772,6 +794,7 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
         D.count = reinterpret_cast<unsigned long long*>(p);
     }
     h->launches = 0;
+    h->defer_clear = -1;
     h->graph = nullptr;
     h->graph_len = 0;
     h->w_filled = -1;
@@ -864,9 +887,10 @@ int hfv_mh2d_advance(hfv_mh2d* h, long long launches, int use_graph, void* strea
             h->graph = nullptr;
             hipGraph_t graph;
             if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return HFV_EDEVICE;
-            const long long saved = h->launches;
+            const long long saved = h->launches, saved_clear = h->defer_clear;
             for (int i = 0; i < GRAPH_LEN; i++) launch_step(h, false, s);
             h->launches = saved;
+            h->defer_clear = saved_clear;
             if (hipStreamEndCapture(s, &graph) != hipSuccess) return HFV_EDEVICE;
             if (hipGraphInstantiate(&h->graph, graph, nullptr, nullptr, 0) != hipSuccess) {
                 (void)hipGraphDestroy(graph);
@@ -878,6 +902,7 @@ int hfv_mh2d_advance(hfv_mh2d* h, long long launches, int use_graph, void* strea
         }
         if (hipGraphLaunch(h->graph, s) != hipSuccess) return HFV_EDEVICE;
         h->launches += GRAPH_LEN;
+        h->defer_clear = h->launches;
         launches -= GRAPH_LEN;
     }
     return hipGetLastError() == hipSuccess ? HFV_OK : HFV_EDEVICE;
