@@ -15,6 +15,7 @@ import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -170,53 +171,95 @@ def sedov_run(lib, n=2048, tf=0.05, use_graph=True):
             "cell_updates_per_s": n * n * st.num_steps / sec}
 
 
-def cpu_baseline(n: int, t_sample: float, steps_sample: int):
-    """The reference's OpenMP build (oracle/_ref) on the same KH problem for a
-    bounded number of steps (tf = time after `steps_sample` steps); falls back
-    to the oracle's OpenMP port when the reference build cannot be loaded."""
+def host_topology():
+    """The host CPU as lscpu reports it (model, sockets, cores per socket,
+    threads per core), and the physical cores this process may run on (its
+    affinity mask, one logical CPU per (package, core) pair)."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            cores.add((open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    return {"model": info.get("Model name"), "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"), "threads_per_core": info.get("Thread(s) per core"),
+            "logical_cpus_allowed": len(cpus), "physical_cores_allowed": len(cores)}
+
+
+def cpu_child(n: int, times):
+    """Child side of cpu_baseline: the reference's OpenMP build (or, when it
+    cannot be loaded, the oracle's OpenMP port) on the KH problem, run from
+    the initial state to each time in `times`; prints one JSON line of
+    (steps, seconds) per run.  Runs in its own process so that the OpenMP
+    placement variables reach the runtime before it starts."""
     import ctypes as C
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
     from hydrofv_amd import abi, problems
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    runs = []
     try:
         ref = abi.bind(C.CDLL(O.REF_OMP_SO))
         kind = "reference"
-    except OSError as e:
-        log(f"cpu_baseline: reference build not loadable ({e}); timing the oracle port")
-        ref = None
-        kind = "port"
-    if ref is not None:
-        sim = problems.kelvin_helmholtz(ref, n)
-        t0 = time.perf_counter()
-        res = sim.launch("muscl_hancock_2d", "riemann_solver_hllc", "monotonized_central", 0.8, t_sample)
-        sec = time.perf_counter() - t0
-        steps = res.num_steps
-        sim.close()
-    else:
-        import hydrofv_amd
-        sim = problems.kelvin_helmholtz(hydrofv_amd.load(), n)
-        import _cases as K
-        grid = K.oracle_grid(sim, "monotonized_central", nthreads=threads)
-        sim.close()
-        t0 = time.perf_counter()
-        r = grid.run(0.8, max_steps=steps_sample)
-        sec = time.perf_counter() - t0
-        steps = r.num_steps
-    value = n * n * steps / sec
-    return {"value": value, "unit": "cell-updates/s", "cores": threads, "kind": kind,
-            "sample": f"KH {n}x{n}, {steps} steps (the 10 CFL-shrink steps), {sec:.1f} s wall, "
-                      f"OMP_NUM_THREADS={threads}",
-            "cpu": host_cpu()}
-
-
-def host_cpu():
-    """Model name and logical CPU count of the host, from /proc/cpuinfo."""
-    try:
-        names = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")]
-        return f"{names[0]} ({len(names)} logical CPUs)" if names else None
     except OSError:
-        return None
+        ref, kind = None, "port"
+    for tf in times:
+        if ref is not None:
+            sim = problems.kelvin_helmholtz(ref, n)
+            t0 = time.perf_counter()
+            res = sim.launch("muscl_hancock_2d", "riemann_solver_hllc", "monotonized_central", 0.8, tf)
+            runs.append((res.num_steps, time.perf_counter() - t0))
+            sim.close()
+        else:
+            import hydrofv_amd
+            import _cases as K
+            sim = problems.kelvin_helmholtz(hydrofv_amd.load(), n)
+            grid = K.oracle_grid(sim, "monotonized_central", nthreads=int(os.environ["OMP_NUM_THREADS"]))
+            sim.close()
+            t0 = time.perf_counter()
+            r = grid.run(0.8, tf=tf)
+            runs.append((r.num_steps, time.perf_counter() - t0))
+    print(json.dumps({"kind": kind, "runs": runs}), flush=True)
+
+
+def cpu_baseline(n: int, t_start: float, t_end: float, shrink_steps: int, window: int):
+    """The reference's OpenMP build (oracle/_ref) on the same KH problem,
+    timed over a window of `window` full-CFL steps: one run to t_start (the
+    `shrink_steps` reduced-CFL steps), one to t_end (the window beyond), and
+    the difference of their wall times.  One OpenMP thread per physical core
+    this process may use, capped by OMP_NUM_THREADS when the host sets it;
+    OMP_PLACES=cores, OMP_PROC_BIND=close."""
+    topo = host_topology()
+    threads = topo["physical_cores_allowed"]
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        threads = min(threads, int(cap))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores", OMP_PROC_BIND="close")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", str(n), repr(t_start), repr(t_end)]
+    out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=1800)
+    if out.returncode != 0:
+        raise RuntimeError(f"cpu baseline child failed ({out.returncode}): {out.stderr[-2000:]}")
+    child = json.loads(out.stdout.strip().splitlines()[-1])
+    (s0, sec0), (s1, sec1) = child["runs"]
+    if s0 != shrink_steps or s1 != shrink_steps + window:
+        raise RuntimeError(f"cpu baseline took {s0} and {s1} steps, expected {shrink_steps} and "
+                           f"{shrink_steps + window}")
+    sec = sec1 - sec0
+    return {"value": n * n * window / sec, "unit": "cell-updates/s", "cores": threads, "kind": child["kind"],
+            "sample": f"KH {n}x{n}: steps {shrink_steps + 1}-{shrink_steps + window} (the {window} full-CFL steps "
+                      f"after the {shrink_steps} shrink steps), {sec:.1f} s wall = run to step "
+                      f"{shrink_steps + window} ({sec1:.1f} s) - run to step {shrink_steps} ({sec0:.1f} s)",
+            "omp": {"OMP_NUM_THREADS": threads, "OMP_PLACES": "cores", "OMP_PROC_BIND": "close"},
+            "cpu": topo}
 
 
 def main():
@@ -235,10 +278,17 @@ def main():
                     help="steps of the 256-instance batched ensemble and of its sequential baseline (0: skip)")
     ap.add_argument("--no-extra-configs", action="store_true",
                     help="skip the Sod 1D and Sedov 2D configuration runs")
+    ap.add_argument("--cpu-window", type=int, default=100,
+                    help="full-CFL steps of the KH CPU baseline's timed window (after the 10 shrink steps)")
+    ap.add_argument("--cpu-child", nargs="+", help=argparse.SUPPRESS)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB once on the device: a "
                          "kernel of known traffic that scales the profiler's FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
+
+    if args.cpu_child:  # the CPU baseline's own process (no GPU)
+        cpu_child(int(args.cpu_child[0]), [float(t) for t in args.cpu_child[1:]])
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this library has no multi-GPU path (--gpus {args.gpus} requested).",
@@ -310,17 +360,22 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        sample_steps = 10
+        # the times after the shrink steps and after the window beyond them,
+        # from the device run (bitwise the reference's, so the reference
+        # reaches them in exactly those step counts)
+        shrink = cfg.cfl_initial_shrink_num_steps
         probe = gpu.MH2D(lib, cfg)
         sim2 = problems.kelvin_helmholtz(lib, n)
         probe.upload(sim2.primitive(), sim2.conserved())
         sim2.close()
         probe.start()
-        probe.advance(sample_steps - 1, use_graph=False)
-        t_sample = probe.status().t
+        probe.advance(shrink - 1, use_graph=False)
+        t_start = probe.status().t
+        probe.advance(args.cpu_window, use_graph=False)
+        t_end = probe.status().t
         probe.close()
         try:
-            cpu = cpu_baseline(n, t_sample, sample_steps)
+            cpu = cpu_baseline(n, t_start, t_end, shrink, args.cpu_window)
             log(f"cpu baseline: {cpu}")
         except Exception as e:  # the GPU figure stands without it
             log(f"cpu baseline failed: {e!r}")
