@@ -158,6 +158,21 @@ __device__ __forceinline__ void load_row_y(const AdjGrid& G, const Fields4& U, i
     wl[3] = w[3];
 }
 
+// A copy of v that the compiler cannot identify with v, so that values
+// computed from it are computed again rather than kept in registers from an
+// earlier computation from v (an empty asm that claims to modify it).
+__device__ __forceinline__ void unshare(double* v, int n)
+{
+    for (int f = 0; f < n; f++) asm volatile("" : "+v"(v[f]));
+}
+
+template <int LIM>
+__device__ __forceinline__ void slopes_only_y(const double* below, const double* mid, const double* above, double* d)
+{
+#pragma unroll
+    for (int f = 0; f < 4; f++) d[f] = limit<LIM>(mid[f] - below[f], above[f] - mid[f]);
+}
+
 template <int LIM>
 __device__ __forceinline__ void slopes_y(const double* below, const double* mid, const double* above, double* d,
                                          double* pa, double* pb)
@@ -190,7 +205,7 @@ __device__ __forceinline__ void predict_y_local(const double* s, const double* d
 }
 
 template <int LIM>
-__global__ void __launch_bounds__(LANES) k_adj_y(AdjGrid G, Fields4 U, Fields4 Lin, Fields4 Lout, int* flags)
+__global__ void __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(2))) k_adj_y(AdjGrid G, Fields4 U, Fields4 Lin, Fields4 Lout, int* flags)
 {
     const int lane = threadIdx.x;
     const int b = blockIdx.x;
@@ -202,41 +217,46 @@ __global__ void __launch_bounds__(LANES) k_adj_y(AdjGrid G, Fields4 U, Fields4 L
     const double gamma = G.gamma, tol = G.tol;
     int flag = 0;
 
-    // rows i-1, i, i+1 of the pipeline (i = y0 - 2 at the start)
-    double Wp[4], Wi[4], Wn[4], mp, mi, mn;
-    double dummy[4];
-    load_row_y(G, U, y0 - 3, cm, &mp, dummy);   // only row i-1's slope input
-    load_row_y(G, U, y0 - 2, cm, &mi, Wi);
-    load_row_y(G, U, y0 - 1, cm, &mn, Wn);
-#pragma unroll
-    for (int f = 0; f < 4; f++) Wp[f] = dummy[f];
-    double di[4], pai[4], pbi[4], loi[4], hii[4];
-    slopes_y<LIM>(Wp, Wi, Wn, di, pai, pbi);
-    predict_y_local(Wi, di, G.cy, sound_speed(gamma, Wi[0], Wi[3]), loi, hii);
-    double lami[4];
-    {
-        const long long k = (long long)wrap(y0 - 2, G.ny, G.g) * G.pitch + cm;
-#pragma unroll
-        for (int f = 0; f < 4; f++) lami[f] = Lin.f[f][k];
-    }
+    // rows i-1, i, i+1 of the pipeline (i = y0 - 2 at the start).  Only the
+    // rows' primitive states and the cotangent sums travel from one
+    // iteration to the next; slopes, limiter partials and face states are
+    // recomputed from them where needed (the same expressions, so the same
+    // values), and a row's mass and incoming cotangent are read again where
+    // they are used, which keeps the pipeline within half the register file.
+    double Wp[4], Wi[4], Wn[4], m_unused;
+    load_row_y(G, U, y0 - 3, cm, &m_unused, Wp);
+    load_row_y(G, U, y0 - 2, cm, &m_unused, Wi);
+    load_row_y(G, U, y0 - 1, cm, &m_unused, Wn);
     double mloi[4] = {0.0, 0.0, 0.0, 0.0};     // cotangent of row i's low face state
     double accp[4] = {0.0, 0.0, 0.0, 0.0};     // mu_W^y of row i-1 (local order)
     double acci[4] = {0.0, 0.0, 0.0, 0.0};     // of row i
 
     for (int i = y0 - 2; i <= y1; i++) {
-        // row i+2 enters; row i+1's slopes and face states
+        // row i+2 enters
         double W2[4], m2;
         load_row_y(G, U, i + 2, cm, &m2, W2);
-        double d1[4], pa1[4], pb1[4], lo1[4], hi1[4];
-        slopes_y<LIM>(Wi, Wn, W2, d1, pa1, pb1);
-        predict_y_local(Wn, d1, G.cy, sound_speed(gamma, Wn[0], Wn[3]), lo1, hi1);
-        double lam1[4];
+        double lami[4], lam1[4];
         {
-            const long long k = (long long)wrap(i + 1, G.ny, G.g) * G.pitch + cm;
+            const long long k = (long long)wrap(i, G.ny, G.g) * G.pitch + cm;
+            const long long k1 = (long long)wrap(i + 1, G.ny, G.g) * G.pitch + cm;
 #pragma unroll
-            for (int f = 0; f < 4; f++) lam1[f] = Lin.f[f][k];
+            for (int f = 0; f < 4; f++) {
+                lami[f] = Lin.f[f][k];
+                lam1[f] = Lin.f[f][k1];
+            }
         }
-        // face (i | i+1)
+        // face (i | i+1): row i's high state and row i+1's low state
+        double hii[4], lo1[4];
+        {
+            double d1[4], hi1[4];
+            slopes_only_y<LIM>(Wi, Wn, W2, d1);
+            predict_y_local(Wn, d1, G.cy, sound_speed(gamma, Wn[0], Wn[3]), lo1, hi1);
+        }
+        {
+            double di[4], loi[4];
+            slopes_only_y<LIM>(Wp, Wi, Wn, di);
+            predict_y_local(Wi, di, G.cy, sound_speed(gamma, Wi[0], Wi[3]), loi, hii);
+        }
         double mf[4];
         mf[0] = G.fy * (lam1[0] - lami[0]);
         mf[1] = G.fy * (lam1[2] - lami[2]);
@@ -245,7 +265,13 @@ __global__ void __launch_bounds__(LANES) k_adj_y(AdjGrid G, Fields4 U, Fields4 L
         double ml[4], mr[4];
         const int e = hllc_vjp(gamma, tol, hii, lo1, mf, ml, mr);
         if (active) flag |= adj_bits(e);
-        // row i: predictor and slope cotangents
+        // row i: predictor and slope cotangents, from its slopes computed
+        // again
+        unshare(Wp, 4);
+        unshare(Wi, 4);
+        unshare(Wn, 4);
+        double di[4], pai[4], pbi[4];
+        slopes_y<LIM>(Wp, Wi, Wn, di, pai, pbi);
         double ms[4] = {0.0, 0.0, 0.0, 0.0}, md[4];
         predict_vjp(Wi, di, G.cy, gamma, true, mloi, ml, ms, md);
         double accn[4];
@@ -262,7 +288,7 @@ __global__ void __launch_bounds__(LANES) k_adj_y(AdjGrid G, Fields4 U, Fields4 L
             const double w[4] = {Wp[0], Wp[2], Wp[1], Wp[3]};
             const double mw[4] = {accp[0], accp[2], accp[1], accp[3]};
             double mu[4];
-            c2p_vjp(gamma, G.volume, mp, w, mw, mu);
+            c2p_vjp(gamma, G.volume, U.f[0][k], w, mw, mu);
 #pragma unroll
             for (int f = 0; f < 4; f++) Lout.f[f][k] += mu[f];
         }
@@ -272,18 +298,10 @@ __global__ void __launch_bounds__(LANES) k_adj_y(AdjGrid G, Fields4 U, Fields4 L
             Wp[f] = Wi[f];
             Wi[f] = Wn[f];
             Wn[f] = W2[f];
-            di[f] = d1[f];
-            pai[f] = pa1[f];
-            pbi[f] = pb1[f];
-            hii[f] = hi1[f];
             mloi[f] = mr[f];
-            lami[f] = lam1[f];
             accp[f] = acci[f];
             acci[f] = accn[f];
         }
-        mp = mi;
-        mi = mn;
-        mn = m2;
     }
     if (__ballot(flag != 0)) {
         int all = flag;
