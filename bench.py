@@ -274,8 +274,9 @@ def main():
                     help="plain launches instead of captured graphs (the KH and Sedov legs)")
     ap.add_argument("--adjoint-solves", type=int, default=32,
                     help="gradient solves of the 1024^2, 500-step adjoint workload (0: skip)")
-    ap.add_argument("--ensemble-steps", type=int, default=96,
-                    help="steps of the 256-instance batched ensemble and of its sequential baseline (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=500,
+                    help="steps of the 256-instance batched ensemble and of its sequential baseline, from the "
+                         "initial state (the workload's 500 by default; 0: skip)")
     ap.add_argument("--no-extra-configs", action="store_true",
                     help="skip the Sod 1D and Sedov 2D configuration runs")
     ap.add_argument("--cpu-window", type=int, default=100,

@@ -107,8 +107,11 @@ struct hfv_ens {
     int2* dtiles;
     int ntiles;
     double* slab;
+    size_t slab_bytes;
     Ctrl* ctrl;
     long long launches;
+    double* saved;           // hfv_ens_checkpoint's copy of the slab (allocated at first use)
+    long long saved_launches;
     hipGraphExec_t graph;
     int graph_phase;
     hipStream_t graph_stream;
@@ -248,6 +251,9 @@ int hfv_ens_create(int count, const hfv_mh2d_config* cfgs, hfv_ens** out)
         total += (size_t)G.pitch * G.ty * 12;
     }
     const size_t bytes = total * sizeof(double) + (size_t)count * 3 * sizeof(Ctrl) + 256;
+    e->slab_bytes = bytes;
+    e->saved = nullptr;
+    e->saved_launches = -1;
     if (hipMalloc(&e->slab, bytes) != hipSuccess) {
         delete e;
         return HFV_ENOMEM;
@@ -297,9 +303,30 @@ void hfv_ens_destroy(hfv_ens* e)
     (void)hipStreamSynchronize(e->own);
     (void)hipStreamDestroy(e->own);
     free_tiles(e);
+    if (e->saved) (void)hipFree(e->saved);
     (void)hipFree(e->dinst);
     (void)hipFree(e->slab);
     delete e;
+}
+
+// Save (restore = 0) or restore (1) the whole device state: every
+// instance's fields and time control, and the step count that selects the
+// controller slot.
+int hfv_ens_checkpoint(hfv_ens* e, int restore, void* stream)
+{
+    hipStream_t s = stream ? (hipStream_t)stream : e->own;
+    if (restore) {
+        if (!e->saved || e->saved_launches < 0) return HFV_EINVAL;
+        if (hipMemcpyAsync(e->slab, e->saved, e->slab_bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            return HFV_EDEVICE;
+        e->launches = e->saved_launches;
+        return HFV_OK;
+    }
+    if (!e->saved && hipMalloc(&e->saved, e->slab_bytes) != hipSuccess) return HFV_ENOMEM;
+    if (hipMemcpyAsync(e->saved, e->slab, e->slab_bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return HFV_EDEVICE;
+    e->saved_launches = e->launches;
+    return HFV_OK;
 }
 
 int hfv_ens_set_strip_rows(hfv_ens* e, const int* rows)

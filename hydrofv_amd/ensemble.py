@@ -27,6 +27,7 @@ PROTOTYPES = {
     "hfv_ens_advance": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_void_p]),
     "hfv_ens_status": (C.c_int, [C.c_void_p, _P(RunStatus), C.c_void_p]),
     "hfv_ens_download": (C.c_int, [C.c_void_p, C.c_int, _P(gpu.dp), _P(gpu.dp), C.c_void_p]),
+    "hfv_ens_checkpoint": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
 }
 
 
@@ -96,6 +97,14 @@ class Ensemble:
         check(self.lib, self.lib.hfv_ens_status(self.h, st, stream), "status")
         return st
 
+    def checkpoint(self, stream=None):
+        """Keep a device copy of the whole state (fields and time control)."""
+        check(self.lib, self.lib.hfv_ens_checkpoint(self.h, 0, stream), "checkpoint")
+
+    def restore(self, stream=None):
+        """Put the state kept by checkpoint() back."""
+        check(self.lib, self.lib.hfv_ens_checkpoint(self.h, 1, stream), "restore")
+
     def download(self, i, stream=None):
         c = self.cfgs[i]
         shape = (c.num_cells_y + 2 * c.num_ghost_cells_side, c.num_cells_x + 2 * c.num_ghost_cells_side)
@@ -125,30 +134,40 @@ def strip_rule(sides, kind, value):
     return np.clip(np.rint(sides / value), 8, 512).astype(int)
 
 
-def autotune(ens: Ensemble, sides, trial_launches=8, log=None, stream=None):
+def autotune(ens: Ensemble, sides, stream, trial_launches=48, log=None):
     """Pick the strip heights and the HLLC code form (hfv_mh2d_config.hllc_form)
-    that minimise the ensemble step time.  Each candidate is timed over
-    `trial_launches` steps continuing from the current device state (the
-    step cost hardly depends on the state; both forms of a strip rule are
-    timed back to back); the batch's initial state is uploaded once
-    afterwards.  Returns ((form, kind, value, seconds per step, rows), all
-    results)."""
+    that minimise the ensemble step time.  Every candidate starts from the
+    same state (the device copy checkpoint() keeps of the state the call
+    finds), runs its first step and one untimed graph replay of
+    `trial_launches` steps (which captures the graph), is put back to that
+    state, and is then timed with HIP events on `stream` (a torch stream)
+    over the first step and one replay of the same graph.  The state is
+    restored afterwards.  Returns ((form, kind, value, seconds per step,
+    rows), all results)."""
+    import torch
+
+    sh = stream.cuda_stream
     rules = [("fixed", v) for v in (16, 32, 64)] + [("prop", d) for d in (4, 8, 16, 32)]
     forms = (gpu.HLLC_SELECTS, gpu.HLLC_BRANCHES) if ens.cfgs[0].riemann_solver == 2 else (gpu.HLLC_SELECTS,)
     best = None
     results = []
-    ens.start(stream)
+    ens.checkpoint(sh)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for kind, value in rules:
         rows = strip_rule(sides, kind, value)
         ens.set_strip_rows(rows)
         for form in forms:
             ens.set_hllc_form(form)
-            ens.advance(2, use_graph=False, stream=stream)
-            ens.status(stream)
-            t0 = time.perf_counter()
-            ens.advance(trial_launches, use_graph=False, stream=stream)
-            ens.status(stream)
-            dt = (time.perf_counter() - t0) / trial_launches
+            ens.restore(sh)
+            ens.start(sh)
+            ens.advance(trial_launches, use_graph=True, stream=sh)
+            ens.restore(sh)
+            e0.record(stream)
+            ens.start(sh)
+            ens.advance(trial_launches, use_graph=True, stream=sh)
+            e1.record(stream)
+            e1.synchronize()
+            dt = e0.elapsed_time(e1) / 1e3 / (trial_launches + 1)
             results.append((form, kind, value, dt, ens.num_tiles))
             if log:
                 log(f"autotune form={FORM_NAMES[form]} {kind}={value}: {dt * 1e3:.3f} ms/step, {ens.num_tiles} tiles")
@@ -156,21 +175,21 @@ def autotune(ens: Ensemble, sides, trial_launches=8, log=None, stream=None):
                 best = (form, kind, value, dt, rows)
     ens.set_strip_rows(best[4])
     ens.set_hllc_form(best[0])
-    if ens.init is not None:
-        ens.upload(ens.init, stream)
+    ens.restore(sh)
     return best, results
 
 
 FORM_NAMES = {gpu.HLLC_SELECTS: "selects", gpu.HLLC_BRANCHES: "branches"}
 
 
-def benchmark(lib, count=256, seed=2, steps=96, log=None):
+def benchmark(lib, count=256, seed=2, steps=500, log=None):
     """Aggregate cell-updates/s of the batched engine (autotuned strips and
-    HLLC code form, one launch per step, hipGraph replays) against the same
-    instances run one after another on the single-grid engine (the same code
-    form, its own graph replays and default strips), both timed over `steps`
-    steps from the initial state with the data resident on the device."""
-    assert steps % 48 == 0, "whole graph replays only"
+    HLLC code form, one launch per step, hipGraph replays of 48 steps)
+    against the same instances run one after another on the single-grid
+    engine (the same code form, its own graph replays and default strips).
+    Both are timed with HIP events over `steps` steps from the initial state
+    (the first step, which reads the primitive state, included), data
+    resident on the device, graphs captured beforehand."""
     import torch
 
     sides, cfgs, init = build_instances(lib, count=count, seed=seed, steps=-1)
@@ -178,20 +197,23 @@ def benchmark(lib, count=256, seed=2, steps=96, log=None):
     stream = torch.cuda.Stream()
     sh = stream.cuda_stream
     ens = Ensemble(lib, cfgs, init, stream=sh)
-    best, _ = autotune(ens, sides, log=log, stream=sh)
+    best, _ = autotune(ens, sides, stream, log=log)
+    # capture the replays the timed run takes, then start over from the
+    # initial state
+    ens.checkpoint(sh)
     ens.start(sh)
-    ens.advance(steps - 1, use_graph=True, stream=sh)     # warm-up, graph capture
-    ens.upload(init, sh)
-    ens.start(sh)
+    ens.advance(steps - 1, use_graph=True, stream=sh)
+    ens.restore(sh)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    ens.advance(steps, use_graph=True, stream=sh)
+    ens.start(sh)
+    ens.advance(steps - 1, use_graph=True, stream=sh)
     e1.record(stream)
     torch.cuda.synchronize()
     t_batch = e0.elapsed_time(e1) / 1e3
     st = ens.status(sh)
-    assert all(s.err_kind == 0 and s.num_steps == steps + 1 for s in st), "ensemble run failed"
+    assert all(s.err_kind == 0 and s.num_steps == steps for s in st), "ensemble run failed"
     ens.close()
 
     singles = []
@@ -200,18 +222,21 @@ def benchmark(lib, count=256, seed=2, steps=96, log=None):
         m = gpu.MH2D(lib, c)
         m.upload(W, U, stream=sh)
         m.start(stream=sh)
-        # warm-up and graph capture; after 1 + steps launches (steps % 3 == 0)
-        # the timed replays use the captured controller phase
-        m.advance(steps, use_graph=True, stream=sh)
+        # graph capture for the controller phase the timed run replays at
+        m.advance(48, use_graph=True, stream=sh)
+        m.upload(W, U, stream=sh)
         singles.append(m)
     torch.cuda.synchronize()
     e0.record(stream)
     for m in singles:
-        m.advance(steps, use_graph=True, stream=sh)
+        m.start(stream=sh)
+        m.advance(steps - 1, use_graph=True, stream=sh)
     e1.record(stream)
     torch.cuda.synchronize()
     t_seq = e0.elapsed_time(e1) / 1e3
     for m in singles:
+        st = m.status(stream=sh)
+        assert st.err_kind == 0 and st.num_steps == steps, "sequential run failed"
         m.close()
     batch = cells * steps / t_batch
     seq = cells * steps / t_seq

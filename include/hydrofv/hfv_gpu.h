@@ -182,6 +182,12 @@ int hfv_ens_start(hfv_ens *e, void *stream);
 int hfv_ens_advance(hfv_ens *e, long long launches, int use_graph, void *stream);
 /* st[count]: state of every instance. */
 int hfv_ens_status(hfv_ens *e, hfv_run_status *st, void *stream);
+/* restore = 0: keep a device copy of every instance's state and time control
+ * (allocated at the first call, as large as the ensemble's own buffers);
+ * restore = 1: put that copy back (HFV_EINVAL before any save).  Both are
+ * stream-ordered device-to-device copies; the autotuner starts every
+ * candidate from the same state this way. */
+int hfv_ens_checkpoint(hfv_ens *e, int restore, void *stream);
 int hfv_ens_download(hfv_ens *e, int inst, double *const W[4], double *const U[4], void *stream);
 
 /* ---- 1D: muscl_hancock_1d (integrator_muscl_hancock.c:26-383) ---------- */

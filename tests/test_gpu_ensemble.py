@@ -80,13 +80,37 @@ def _sim_like(lib, n, init):
     return s
 
 
+def test_checkpoint_restore_replays_the_same_run(gpulib, small_batch):
+    """A state kept by checkpoint() and put back by restore() continues
+    exactly as the first time: the same steps, times and bits, through graph
+    replays and plain launches alike."""
+    sides, cfgs, init = small_batch
+    ens = ensemble.Ensemble(gpulib, cfgs, init)
+    with pytest.raises(RuntimeError):
+        ens.restore()  # nothing kept yet
+    ens.start()
+    ens.advance(5, use_graph=False)
+    ens.checkpoint()
+    ens.advance(50, use_graph=True)
+    st1 = ens.status()
+    U1 = [np.stack(ens.download(i)[1]) for i in range(len(cfgs))]
+    ens.restore()
+    ens.advance(50, use_graph=True)
+    st2 = ens.status()
+    for i in range(len(cfgs)):
+        assert (st1[i].num_steps, st1[i].t, st1[i].dt) == (st2[i].num_steps, st2[i].t, st2[i].dt), i
+        assert np.array_equal(U1[i].view(np.uint64), np.stack(ens.download(i)[1]).view(np.uint64)), i
+    ens.close()
+
+
 def test_autotune_picks_a_timed_candidate(gpulib, small_batch):
     """The autotuner leaves the batch in its fastest configuration (strips
     and code form) with the initial state restored, so a run after it
     equals a plain run."""
+    import torch
     sides, cfgs, init = small_batch
     ens = ensemble.Ensemble(gpulib, cfgs, init)
-    best, results = ensemble.autotune(ens, sides, trial_launches=2)
+    best, results = ensemble.autotune(ens, sides, torch.cuda.Stream(), trial_launches=2)
     assert len(results) == 14 and {r[0] for r in results} == {0, 1}
     assert best[3] == min(r[3] for r in results)
     ens.start()

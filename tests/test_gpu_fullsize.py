@@ -95,8 +95,9 @@ def test_ensemble_256_autotuned_equals_single_runs(gpulib):
     # no step limit, as in the benchmark (the autotuner times live steps);
     # exactly `steps` launches below
     sides, cfgs, init = ensemble.build_instances(gpulib, count=256, seed=2, steps=-1)
+    import torch
     ens = ensemble.Ensemble(gpulib, cfgs, init)
-    best, _ = ensemble.autotune(ens, sides, trial_launches=2)
+    best, _ = ensemble.autotune(ens, sides, torch.cuda.Stream(), trial_launches=2)
     print(f"ensemble: tiles {best[1]}={best[2]}, form {ensemble.FORM_NAMES[best[0]]}, {ens.num_tiles} tiles",
           flush=True)
     ens.start()
