@@ -470,6 +470,21 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     __shared__ double trrs_q[6 * LANES];
     __shared__ double pow_lds[POW_TABS_LEN];
     if constexpr (SOLVER == HLLC) stage_pow_tables(pow_lds, lane);
+    constexpr int AHEAD = (LIM == FIRST_ORDER) ? 1 : 2;  // the row a step's iteration r reads
+    // The conserved states of the last AHEAD + 1 rows loaded, each lane its
+    // own column (slot q % (AHEAD + 1) holds row q): a row's update takes its
+    // conserved state from here, as loaded for its c2p AHEAD iterations
+    // earlier, instead of holding it in registers or loading it again.  For an
+    // updated cell (interior row and column) the loaded cell is the cell
+    // itself.  A lane reads only what it wrote, so no barrier is needed.
+    __shared__ double urows[(AHEAD + 1) * 4 * LANES];
+    const auto stash_row = [&](int q, const RawCell& rc) {
+        if constexpr (!WMODE) {
+            double* u = urows + (q % (AHEAD + 1)) * 4 * LANES + lane;
+#pragma unroll
+            for (int f = 0; f < 4; f++) u[f * LANES] = rc.v[f];
+        }
+    };
 
     // ---- prologue: rows y0-2 .. y0+1 (first order: y0-1 .. y0) ----------
     double wa[4], wb[4], wc[4];
@@ -477,16 +492,20 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     double cxr_row = 0.0, cxr_next = 0.0;  // cx / rho of rows r and r+1
     double yt[4], fyb[4];
     // The next row's stored values, loaded an iteration ahead.  A row's update
-    // loads the row's conserved state again, at the top of its iteration and
-    // before that prefetch: the vector-memory counter completes in order, so
-    // the update then waits for its own loads only (they hit in cache), and
-    // no lane holds the conserved states of the rows between their c2p and
-    // their update.
+    // takes the row's conserved state from urows (WMODE: loads it again, at
+    // the top of its iteration and before that prefetch, since the
+    // vector-memory counter completes in order), so no lane holds the
+    // conserved states of the rows between their c2p and their update.
     RawCell next;
     if constexpr (LIM == FIRST_ORDER) {
         double wm1[4];
         load_prim<WMODE>(G, U, W, y0 - 1, col, wm1, rvol, vol_ok);
-        load_prim<WMODE>(G, U, W, y0, col, wa, rvol, vol_ok);
+        {
+            RawCell rc;
+            load_raw<WMODE>(G, U, W, y0, col, rc);
+            stash_row(y0, rc);
+            raw_to_prim<WMODE>(G, rc, wa, rvol, vol_ok);
+        }
 #pragma unroll
         for (int f = 0; f < 4; f++) {
             wb[f] = wa[f];  // unused
@@ -503,8 +522,15 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         double wm2[4], wm1[4];
         load_prim<WMODE>(G, U, W, y0 - 2, col, wm2, rvol, vol_ok);
         load_prim<WMODE>(G, U, W, y0 - 1, col, wm1, rvol, vol_ok);
-        load_prim<WMODE>(G, U, W, y0, col, wa, rvol, vol_ok);
-        load_prim<WMODE>(G, U, W, y0 + 1, col, wb, rvol, vol_ok);
+        {
+            RawCell rc;
+            load_raw<WMODE>(G, U, W, y0, col, rc);
+            stash_row(y0, rc);
+            raw_to_prim<WMODE>(G, rc, wa, rvol, vol_ok);
+            load_raw<WMODE>(G, U, W, y0 + 1, col, rc);
+            stash_row(y0 + 1, rc);
+            raw_to_prim<WMODE>(G, rc, wb, rvol, vol_ok);
+        }
         double lo[4], hi_m1[4], lo0[4], unused;
         ypredict<LIM>(wm2, wm1, wa, cy, cx, gamma, lo, hi_m1, &unused);         // row y0-1: top face state
         a_row = ypredict<LIM>(wm1, wa, wb, cy, cx, gamma, lo0, yt, &cxr_row);   // row y0: bottom and top face states
@@ -529,7 +555,6 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         V.f[3][k] = U.f[3][k] - fyb[3] * dt * G.area_y;
     }
 
-    constexpr int AHEAD = (LIM == FIRST_ORDER) ? 1 : 2;  // the row a step's iteration r reads
     load_raw<WMODE>(G, U, W, y0 + AHEAD, col, next);
     for (int r = y0; r < y1; r++) {
         // y: row r+1's face states (the flux through the top face of row r
@@ -537,11 +562,14 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         // neighbours taken by shuffles
         double ylo[4], yhi[4], xlo[4], xhi[4];
         raw_to_prim<WMODE>(G, next, wc, rvol, vol_ok);  // row r + AHEAD
+        stash_row(r + AHEAD, next);
         const long long k = (long long)r * G.pitch + col;
-        double uk[4];  // this row's conserved state, for its update
-        if (out_col) {
+        double uk[4];  // this row's conserved state, for its update (WMODE: loaded again)
+        if constexpr (WMODE) {
+            if (out_col) {
 #pragma unroll
-            for (int f = 0; f < 4; f++) uk[f] = U.f[f][k];
+                for (int f = 0; f < 4; f++) uk[f] = U.f[f][k];
+            }
         }
         if (r + 1 < y1) load_raw<WMODE>(G, U, W, r + 1 + AHEAD, col, next);
         if constexpr (LIM == FIRST_ORDER) {
@@ -657,6 +685,11 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         if (deferred) {
             // updated by k_update_deferred
         } else if (out_col) {
+            if constexpr (!WMODE) {
+                const double* us = urows + (r % (AHEAD + 1)) * 4 * LANES + lane;
+#pragma unroll
+                for (int f = 0; f < 4; f++) uk[f] = us[f * LANES];
+            }
             double u[4];
 #pragma unroll
             for (int f = 0; f < 4; f++) {

@@ -142,7 +142,7 @@ def autotune(ens: Ensemble, sides, stream, trial_launches=48, log=None):
     `trial_launches` steps (which captures the graph), is put back to that
     state, and is then timed with HIP events on `stream` (a torch stream)
     over the first step and one replay of the same graph.  The state is
-    restored afterwards.  Returns ((form, kind, value, seconds per step,
+    restored afterwards, and `stream` synchronised.  Returns ((form, kind, value, seconds per step,
     rows), all results)."""
     import torch
 
@@ -176,6 +176,7 @@ def autotune(ens: Ensemble, sides, stream, trial_launches=48, log=None):
     ens.set_strip_rows(best[4])
     ens.set_hllc_form(best[0])
     ens.restore(sh)
+    stream.synchronize()  # the caller may go on on another stream
     return best, results
 
 
