@@ -81,6 +81,10 @@ __global__ void __launch_bounds__(LANES) k_adj_x(AdjGrid G, Fields4 U, Fields4 L
     const bool face_used = lane >= 1 && near != 0ull;
     const int r0 = G.g + sy * G.rows_x, r1 = min(r0 + G.rows_x, G.g + G.ny);
     const double gamma = G.gamma, tol = G.tol;
+    // c2p and sound speeds as the forward step takes them (c2p_2d_rn,
+    // sound_speed_rn: the same bits as c2p_2d and sound_speed, cheaper)
+    const double rvol = rcp_core(G.volume);
+    const bool vol_ok = div_core_ok(G.volume);
     int flag = 0;
 
     for (int r = r0; r < r1; r++) {
@@ -91,7 +95,7 @@ __global__ void __launch_bounds__(LANES) k_adj_x(AdjGrid G, Fields4 U, Fields4 L
             u[f] = U.f[f][k];
             lam[f] = Lin.f[f][k];
         }
-        c2p_2d(gamma, G.volume, u[0], u[1], u[2], u[3], w);
+        c2p_2d_rn(gamma, G.volume, rvol, vol_ok, u[0], u[1], u[2], u[3], w);
         double d[4], pa[4], pb[4];
 #pragma unroll
         for (int f = 0; f < 4; f++) {
@@ -101,7 +105,7 @@ __global__ void __launch_bounds__(LANES) k_adj_x(AdjGrid G, Fields4 U, Fields4 L
             limit_partials<LIM>(a, bb, &pa[f], &pb[f]);
         }
         double xlo[4], xhi[4];
-        predict_x(w, d, G.cx, sound_speed(gamma, w[0], w[3]), xlo, xhi);
+        predict_x(w, d, G.cx, sound_speed_rn(gamma, w[0], w[3]), xlo, xhi);
         // the face at the lane's left: (left neighbour's xhi | own xlo)
         double hl[4], mf[4];
 #pragma unroll
@@ -145,12 +149,15 @@ __global__ void __launch_bounds__(LANES) k_adj_x(AdjGrid G, Fields4 U, Fields4 L
 
 // y pass ---------------------------------------------------------------------
 // Sweep-local order for the y sweep: (rho, v, u, p); fluxes (mass, my, mx, E).
-__device__ __forceinline__ void load_row_y(const AdjGrid& G, const Fields4& U, int q, int cm, double* m, double* wl)
+// rvol, vol_ok: rcp_core(G.volume), div_core_ok(G.volume) (c2p_2d_rn); every
+// lane of the wavefront calls this
+__device__ __forceinline__ void load_row_y(const AdjGrid& G, const Fields4& U, int q, int cm, double* m, double* wl,
+                                           double rvol, bool vol_ok)
 {
     const long long k = (long long)wrap(q, G.ny, G.g) * G.pitch + cm;
     double w[4];
     const double u0 = U.f[0][k];
-    c2p_2d(G.gamma, G.volume, u0, U.f[1][k], U.f[2][k], U.f[3][k], w);
+    c2p_2d_rn(G.gamma, G.volume, rvol, vol_ok, u0, U.f[1][k], U.f[2][k], U.f[3][k], w);
     *m = u0;
     wl[0] = w[0];
     wl[1] = w[2];
@@ -223,10 +230,12 @@ __global__ void __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(2)))
     // recomputed from them where needed (the same expressions, so the same
     // values), and a row's mass and incoming cotangent are read again where
     // they are used, which keeps the pipeline within half the register file.
+    const double rvol = rcp_core(G.volume);
+    const bool vol_ok = div_core_ok(G.volume);
     double Wp[4], Wi[4], Wn[4], m_unused;
-    load_row_y(G, U, y0 - 3, cm, &m_unused, Wp);
-    load_row_y(G, U, y0 - 2, cm, &m_unused, Wi);
-    load_row_y(G, U, y0 - 1, cm, &m_unused, Wn);
+    load_row_y(G, U, y0 - 3, cm, &m_unused, Wp, rvol, vol_ok);
+    load_row_y(G, U, y0 - 2, cm, &m_unused, Wi, rvol, vol_ok);
+    load_row_y(G, U, y0 - 1, cm, &m_unused, Wn, rvol, vol_ok);
     double mloi[4] = {0.0, 0.0, 0.0, 0.0};     // cotangent of row i's low face state
     double accp[4] = {0.0, 0.0, 0.0, 0.0};     // mu_W^y of row i-1 (local order)
     double acci[4] = {0.0, 0.0, 0.0, 0.0};     // of row i
@@ -234,7 +243,7 @@ __global__ void __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(2)))
     for (int i = y0 - 2; i <= y1; i++) {
         // row i+2 enters
         double W2[4], m2;
-        load_row_y(G, U, i + 2, cm, &m2, W2);
+        load_row_y(G, U, i + 2, cm, &m2, W2, rvol, vol_ok);
         double lami[4], lam1[4];
         {
             const long long k = (long long)wrap(i, G.ny, G.g) * G.pitch + cm;
@@ -250,12 +259,12 @@ __global__ void __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(2)))
         {
             double d1[4], hi1[4];
             slopes_only_y<LIM>(Wi, Wn, W2, d1);
-            predict_y_local(Wn, d1, G.cy, sound_speed(gamma, Wn[0], Wn[3]), lo1, hi1);
+            predict_y_local(Wn, d1, G.cy, sound_speed_rn(gamma, Wn[0], Wn[3]), lo1, hi1);
         }
         {
             double di[4], loi[4];
             slopes_only_y<LIM>(Wp, Wi, Wn, di);
-            predict_y_local(Wi, di, G.cy, sound_speed(gamma, Wi[0], Wi[3]), loi, hii);
+            predict_y_local(Wi, di, G.cy, sound_speed_rn(gamma, Wi[0], Wi[3]), loi, hii);
         }
         double mf[4];
         mf[0] = G.fy * (lam1[0] - lami[0]);

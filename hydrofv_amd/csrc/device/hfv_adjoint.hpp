@@ -9,6 +9,11 @@
 // tol clamp of the guess), with the branch taken by the forward evaluation.
 // The vacuum branch of HLLC has no derivative here: its occurrence is
 // reported and its contribution is zero.
+// The forward quantities that choose a branch are evaluated with the
+// forward's own expressions, so the branch is the forward's.  Quantities
+// that enter only the derivatives divide by a denominator through its
+// reciprocal, computed once: the derivatives are then within a few ulps of
+// the exact quotients' (cheaper than one correctly rounded division each).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -48,12 +53,26 @@ HFV_HD __forceinline__ int hllc_vjp(double gamma, double tol, const double* wl, 
         gb = 1;
         h = 0.5 * (gamma - 1.0);
         z = h / gamma;
-        pzl = pow_pos(pl, z);
-        pzr = pow_pos(pr, z);
         tnum = al + ar - h * (ur - ul);
-        tden = al / pzl + ar / pzr;
-        X = tnum / tden;
-        ps = pow_pos(X, gamma / h);
+        HllcFace F;
+        F.wl[3] = pl;
+        F.wr[3] = pr;
+        F.al = al;
+        F.ar = ar;
+        F.ps = tnum;
+        if (trrs_settle_direct(gamma, tol, F)) {
+            // p* lies below both pressures (as the forward establishes it):
+            // q_L = q_R = 1, so the guess gets no cotangent and its powers
+            // are not needed; finite stand-ins keep the zero terms zero
+            ps = F.ps;
+            pzl = pzr = tden = X = 1.0;
+        } else {
+            pzl = pow_pos(pl, z);
+            pzr = pow_pos(pr, z);
+            tden = al / pzl + ar / pzr;
+            X = tnum / tden;
+            ps = pow_pos(X, gamma / h);
+        }
     } else {
         gb = 2;
         Al = coeff_A(gamma, rl);
@@ -99,9 +118,11 @@ HFV_HD __forceinline__ int hllc_vjp(double gamma, double tol, const double* wl, 
         const double s = left ? sl : sr;
         const double w = s - u;
         const double D = s - sm;
-        const double rs = r * w / D;
-        const double T = sm + p / (r * w);
-        const double Q = e / r + (sm - u) * T;
+        const double ir = 1.0 / r, iD = 1.0 / D, irw = 1.0 / (r * w);
+        const double iw = r * irw;
+        const double rs = r * w * iD;
+        const double T = sm + p * irw;
+        const double Q = e * ir + (sm - u) * T;
         const double es = rs * Q;
         double m_rs = 0.0, m_es = 0.0, m_w = 0.0, m_D = 0.0;
         // F0 = fm + s (rs - r)
@@ -133,19 +154,19 @@ HFV_HD __forceinline__ int hllc_vjp(double gamma, double tol, const double* wl, 
         // es = rs Q, Q = e/r + (sm - u) T, T = sm + p/(r w)
         m_rs += m_es * Q;
         const double m_Q = m_es * rs;
-        m_e += m_Q / r;
-        m_r -= m_Q * e / (r * r);
+        m_e += m_Q * ir;
+        m_r -= m_Q * e * ir * ir;
         m_sm += m_Q * T;
         m_u -= m_Q * T;
         const double m_T = m_Q * (sm - u);
         m_sm += m_T;
-        m_p += m_T / (r * w);
-        m_r -= m_T * p / (r * r * w);
-        m_w -= m_T * p / (r * w * w);
+        m_p += m_T * irw;
+        m_r -= m_T * p * irw * ir;
+        m_w -= m_T * p * irw * iw;
         // rs = r w / D
-        m_r += m_rs * w / D;
-        m_w += m_rs * r / D;
-        m_D -= m_rs * rs / D;
+        m_r += m_rs * w * iD;
+        m_w += m_rs * r * iD;
+        m_D -= m_rs * rs * iD;
         // w = s - u, D = s - sm
         m_s += m_w + m_D;
         m_u -= m_w;
@@ -155,7 +176,7 @@ HFV_HD __forceinline__ int hllc_vjp(double gamma, double tol, const double* wl, 
     m_r += m_fm * u + m_e * 0.5 * (u * u + v * v);
     m_u += m_fm * r + m_e * r * u;
     m_v += m_e * r * v;
-    m_p += m_e / (gamma - 1.0);
+    m_p += m_e * (1.0 / (gamma - 1.0));
     double* mk = left ? ml : mr;
     mk[0] += m_r;
     mk[1] += m_u;
@@ -165,8 +186,9 @@ HFV_HD __forceinline__ int hllc_vjp(double gamma, double tol, const double* wl, 
 
     // ---- s (= sl or sr) and sm back to the inputs ------------------------
     double m_sl = left ? m_s : 0.0, m_sr = left ? 0.0 : m_s;
-    const double m_num = m_sm / den;
-    const double m_den = -m_sm * sm / den;
+    const double iden = 1.0 / den;
+    const double m_num = m_sm * iden;
+    const double m_den = -m_sm * sm * iden;
     ml[3] -= m_num;
     mr[3] += m_num;
     ml[0] += m_num * ul * (sl - ul) + m_den * (sl - ul);
@@ -181,13 +203,16 @@ HFV_HD __forceinline__ int hllc_vjp(double gamma, double tol, const double* wl, 
     double m_al = -m_sl * ql, m_ar = m_sr * qr;
     const double m_ql = -m_sl * al, m_qr = m_sr * ar;
     double m_ps = 0.0;
+    const double ipl = 1.0 / pl, ipr = 1.0 / pr;
     if (shl) {
-        m_ps += m_ql * kq / (2.0 * ql * pl);
-        ml[3] -= m_ql * kq * ps / (2.0 * ql * pl * pl);
+        const double t = m_ql * kq / (2.0 * ql) * ipl;
+        m_ps += t;
+        ml[3] -= t * ps * ipl;
     }
     if (shr) {
-        m_ps += m_qr * kq / (2.0 * qr * pr);
-        mr[3] -= m_qr * kq * ps / (2.0 * qr * pr * pr);
+        const double t = m_qr * kq / (2.0 * qr) * ipr;
+        m_ps += t;
+        mr[3] -= t * ps * ipr;
     }
     // ---- pressure guess back to the inputs ------------------------------
     double m_ppv = 0.0;
@@ -196,28 +221,31 @@ HFV_HD __forceinline__ int hllc_vjp(double gamma, double tol, const double* wl, 
             m_ppv = m_ps;
         } else if (gb == 1) {
             const double m_X = m_ps * ps / (z * X);
-            const double m_tn = m_X / tden;
-            const double m_td = -m_X * X / tden;
-            m_al += m_tn + m_td / pzl;
-            m_ar += m_tn + m_td / pzr;
+            const double itd = 1.0 / tden, ipzl = 1.0 / pzl, ipzr = 1.0 / pzr;
+            const double m_tn = m_X * itd;
+            const double m_td = -m_X * X * itd;
+            m_al += m_tn + m_td * ipzl;
+            m_ar += m_tn + m_td * ipzr;
             mr[1] -= m_tn * h;
             ml[1] += m_tn * h;
-            ml[3] -= m_td * al * z / (pzl * pl);
-            mr[3] -= m_td * ar * z / (pzr * pr);
+            ml[3] -= m_td * al * z * ipzl * ipl;
+            mr[3] -= m_td * ar * z * ipzr * ipr;
         } else {
-            const double S = gl + gr;
-            const double m_gl = m_ps * (pl - ps) / S;
-            const double m_gr = m_ps * (pr - ps) / S;
-            ml[3] += m_ps * gl / S;
-            mr[3] += m_ps * gr / S;
-            mr[1] -= m_ps / S;
-            ml[1] += m_ps / S;
-            const double m_Al = m_gl / (2.0 * gl * Hl), m_Hl = -m_gl * gl / (2.0 * Hl);
-            const double m_Ar = m_gr / (2.0 * gr * Hr), m_Hr = -m_gr * gr / (2.0 * Hr);
+            const double iS = 1.0 / (gl + gr);
+            const double m_gl = m_ps * (pl - ps) * iS;
+            const double m_gr = m_ps * (pr - ps) * iS;
+            ml[3] += m_ps * gl * iS;
+            mr[3] += m_ps * gr * iS;
+            mr[1] -= m_ps * iS;
+            ml[1] += m_ps * iS;
+            const double iHl = 1.0 / Hl, iHr = 1.0 / Hr;
+            const double m_Al = m_gl / (2.0 * gl) * iHl, m_Hl = -m_gl * gl * 0.5 * iHl;
+            const double m_Ar = m_gr / (2.0 * gr) * iHr, m_Hr = -m_gr * gr * 0.5 * iHr;
             ml[0] -= m_Al * Al / rl;
             mr[0] -= m_Ar * Ar / rr;
-            ml[3] += m_Hl * (gamma - 1.0) / (gamma + 1.0);
-            mr[3] += m_Hr * (gamma - 1.0) / (gamma + 1.0);
+            const double gg = (gamma - 1.0) / (gamma + 1.0);
+            ml[3] += m_Hl * gg;
+            mr[3] += m_Hr * gg;
             m_ppv = m_Hl + m_Hr;
         }
     }
@@ -236,10 +264,11 @@ HFV_HD __forceinline__ int hllc_vjp(double gamma, double tol, const double* wl, 
         m_ar -= m_ppv * C8;
     }
     // a = sqrt(gamma p / rho)
-    ml[3] += m_al * al / (2.0 * pl);
-    ml[0] -= m_al * al / (2.0 * rl);
-    mr[3] += m_ar * ar / (2.0 * pr);
-    mr[0] -= m_ar * ar / (2.0 * rr);
+    const double hal = 0.5 * m_al * al, har = 0.5 * m_ar * ar;
+    ml[3] += hal * ipl;
+    ml[0] -= hal / rl;
+    mr[3] += har * ipr;
+    mr[0] -= har / rr;
     return ADJ_OK;
 }
 
@@ -252,17 +281,18 @@ HFV_HD __forceinline__ void predict_vjp(const double* s, const double* d, double
                                             const double* mlo, const double* mhi, double* ms, double* md)
 {
     const double rho = s[0], n = s[1], p = s[3];
+    const double irho = 1.0 / rho;
     const double np_ = 1.0 + c * n, nm = 1.0 - c * n;
     const double dq = quirk ? d[2] : d[1];
     const double m0 = mlo[0] + mhi[0], mn = mlo[1] + mhi[1], mt = mlo[2] + mhi[2], m3 = mlo[3] + mhi[3];
-    ms[0] += m0 * (1.0 - 0.5 * c * dq) + mn * 0.5 * c * d[3] / (rho * rho);
+    ms[0] += m0 * (1.0 - 0.5 * c * dq) + mn * 0.5 * c * d[3] * irho * irho;
     ms[1] += -m0 * 0.5 * c * d[0] + mn * (1.0 - 0.5 * c * d[1]) - mt * 0.5 * c * d[2] - m3 * 0.5 * c * d[3];
     ms[2] += mt;
     ms[3] += m3 * (1.0 - 0.5 * c * gamma * d[1]);
     md[0] = -0.5 * np_ * mlo[0] + 0.5 * nm * mhi[0];
     md[1] = -0.5 * np_ * mlo[1] + 0.5 * nm * mhi[1] - 0.5 * c * gamma * p * m3;
     md[2] = -0.5 * np_ * mlo[2] + 0.5 * nm * mhi[2];
-    md[3] = -0.5 * (c / rho) * mn - 0.5 * np_ * mlo[3] + 0.5 * nm * mhi[3];
+    md[3] = -0.5 * (c * irho) * mn - 0.5 * np_ * mlo[3] + 0.5 * nm * mhi[3];
     if (quirk) md[2] += -0.5 * c * rho * m0;
     else md[1] += -0.5 * c * rho * m0;
 }
@@ -303,10 +333,11 @@ HFV_HD __forceinline__ void c2p_vjp(double gamma, double vol, double m, const do
                                         double* mu)
 {
     const double u = w[1], v = w[2];
-    const double gv = (gamma - 1.0) / vol;
-    mu[0] = mw[0] / vol - mw[1] * u / m - mw[2] * v / m + mw[3] * gv * 0.5 * (u * u + v * v);
-    mu[1] = mw[1] / m - mw[3] * gv * u;
-    mu[2] = mw[2] / m - mw[3] * gv * v;
+    const double ivol = 1.0 / vol, im = 1.0 / m;
+    const double gv = (gamma - 1.0) * ivol;
+    mu[0] = mw[0] * ivol - (mw[1] * u + mw[2] * v) * im + mw[3] * gv * 0.5 * (u * u + v * v);
+    mu[1] = mw[1] * im - mw[3] * gv * u;
+    mu[2] = mw[2] * im - mw[3] * gv * v;
     mu[3] = mw[3] * gv;
 }
 
