@@ -436,6 +436,7 @@ int hfv_ens_status(hfv_ens* e, hfv_run_status* st, void* stream)
         st[i].smax[0] = smax_from_key(x.key[0]);
         st[i].smax[1] = smax_from_key(x.key[1]);
         st[i].finished = (x.err_kind != ERR_NONE || !(x.t < rp.tf) || (rp.max_steps >= 0 && x.num_steps >= rp.max_steps));
+        st[i].bisect_fails = 0;
     }
     return HFV_OK;
 }

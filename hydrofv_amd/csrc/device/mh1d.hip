@@ -37,6 +37,10 @@ struct Arrays1D {
     const double* area;
     const double* mid;
     const double* vol;
+    // exact-solver solves that failed after 5000 bisection iterations where
+    // the reference's loop goes on past a failure (MUSCL-Hancock faces;
+    // random-choice cells), for the host's verbose warnings
+    unsigned long long* bisect_fails;
 };
 
 // boundary.c:411-511 (x_min first, then x_max, per ghost layer)
@@ -201,6 +205,7 @@ __global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays
                 if (S.rc != ERR_NONE) {
                     my_i = i;  // the last failure of the serial loop is reported
                     my_kind = S.rc;
+                    if (S.rc == ERR_EXACT_BISECT) atomicAdd(A.bisect_fails, 1ull);
                 }
             }
             if (my_i >= 0) atomicMax(&fail_last, my_i);
@@ -268,8 +273,11 @@ __global__ void __launch_bounds__(T1D) k_mh1d_run(Grid1D G, RunParams rp, Arrays
             reconstruct(g + i - 1, lo_l, hi_l);
             reconstruct(g + i, lo_r, hi_r);
             double f[3];
-            if (flux<SOLVER, 3>(gamma, G.tol, hi_l, lo_r, f) != ERR_NONE) {
+            const int e = flux<SOLVER, 3>(gamma, G.tol, hi_l, lo_r, f);
+            if (e != ERR_NONE) {
+                // the reference's loop goes on (its error is overwritten)
                 f[0] = f[1] = f[2] = __builtin_nan("");
+                if (e == ERR_EXACT_BISECT) atomicAdd(A.bisect_fails, 1ull);
             }
             for (int q = 0; q < 3; q++) A.fl[q][i] = f[q] * dt;
         }
@@ -541,7 +549,8 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
     const size_t stride = (tx + 2 + 7) / 8 * 8;
     DevBuf<double> buf;
     DevBuf<Ctrl> dctrl;
-    if (!buf.alloc(stride * 18) || !dctrl.alloc(1)) return HFV_ENOMEM;
+    DevBuf<unsigned long long> dfails;
+    if (!buf.alloc(stride * 18) || !dctrl.alloc(1) || !dfails.alloc(1)) return HFV_ENOMEM;
     Arrays1D A;
     for (int f = 0; f < 3; f++) {
         A.w[f] = buf.p + stride * f;
@@ -551,6 +560,8 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
         A.fl[f] = buf.p + stride * (12 + f);
     }
     double* geo = buf.p + stride * 15;
+    A.bisect_fails = dfails.p;
+    if (hipMemsetAsync(dfails.p, 0, sizeof(unsigned long long), s) != hipSuccess) return HFV_EDEVICE;
     A.area = geo;
     A.mid = geo + stride;
     A.vol = geo + 2 * stride;
@@ -607,7 +618,10 @@ int hfv_mh1d_run(const hfv_mh1d_config* cfg, double* const W[3], double* const U
         if (hipMemcpyAsync(W[f], A.w[f], tx * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) return HFV_EDEVICE;
         if (hipMemcpyAsync(U[f], A.u[f], tx * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) return HFV_EDEVICE;
     }
+    unsigned long long fails = 0;
+    if (hipMemcpyAsync(&fails, dfails.p, sizeof fails, hipMemcpyDeviceToHost, s) != hipSuccess) return HFV_EDEVICE;
     if (hipStreamSynchronize(s) != hipSuccess) return HFV_EDEVICE;
+    st->bisect_fails = (long long)fails;
     st->t = c.t;
     st->dt = c.dt;
     st->t_prev = c.t_prev;

@@ -707,6 +707,7 @@ void fill_status(const Ctrl& c, const RunParams& rp, hfv_run_status* st)
     const bool finished =
         c.err_kind != ERR_NONE || !(c.t < rp.tf) || (rp.max_steps >= 0 && c.num_steps >= rp.max_steps);
     st->finished = finished ? 1 : 0;
+    st->bisect_fails = 0;
 }
 
 // This device's address of `scratch`, looked up on the first call on the

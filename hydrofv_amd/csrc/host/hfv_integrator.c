@@ -27,8 +27,30 @@ static ErrorStatus device_error(int rc)
     return WRAP_RAISE_ERROR(FAILURE, msg);
 }
 
-static ErrorStatus run_error(int kind)
+/* The reference's verbose warnings (riemann_solver_exact.c:736-744) for
+ * `solves` exact solves that failed after 5000 bisection iterations: each
+ * warned at every 500th iteration before failing.  A bisection that stops
+ * short of 5000 iterations takes fewer than 500 unless p* lies below 2^-448
+ * of the bracket's width, so the failed solves are the ones that warn. */
+static void bisect_warnings(long long solves, int verbose)
 {
+    if (verbose <= 0) return;
+    char buf[128];
+    for (long long i = 0; i < solves; i++) {
+        for (int c = 500; c <= 5000; c += 500) {
+            snprintf(buf, sizeof buf, "Bisection method did not converge after %d iterations for exact riemann solver.",
+                     c);
+            WRAP_RAISE_WARNING(buf);
+        }
+    }
+}
+
+/* A device failure as the reference reports it; a failed exact solve warns
+ * first when verbose > 0 (the run stops at the first failure, so one solve
+ * warned), unless the device has counted it among st.bisect_fails. */
+static ErrorStatus run_error(int kind, int verbose)
+{
+    if (kind == HFV_ERR_EXACT_BISECT) bisect_warnings(1, verbose);
     switch (kind) {
     case HFV_ERR_DT_ZERO:
         return WRAP_RAISE_ERROR(VALUE_ERROR, "dt is zero.");
@@ -188,7 +210,7 @@ static ErrorStatus storing_begin(Storing *s, const BoundaryConditionParam *bc, c
  * first_order: the first-order Godunov scheme instead of MUSCL-Hancock. */
 static ErrorStatus run_2d(BoundaryConditionParam *bc, System *system, IntegratorParam *ip, StoringParam *sp,
                           SimulationParam *prm, SimulationStatus *status, long long max_steps, double fixed_dt,
-                          int first_order)
+                          int first_order, int verbose)
 {
     char msg[256];
     if (system->coord_sys_flag_ != COORD_SYS_CARTESIAN_2D) {
@@ -317,14 +339,14 @@ static ErrorStatus run_2d(BoundaryConditionParam *bc, System *system, Integrator
     hfv_mh2d_destroy(h);
     if (rc != HFV_OK) return device_error(rc);
     fill_status(status, &st);
-    if (st.err_kind != HFV_ERR_NONE) return run_error(st.err_kind);
+    if (st.err_kind != HFV_ERR_NONE) return run_error(st.err_kind, verbose);
     return make_success_error_status();
 }
 
 /* scheme: 0 MUSCL-Hancock, 1 first-order Godunov, 2 random choice */
 static ErrorStatus run_1d(BoundaryConditionParam *bc, System *system, IntegratorParam *ip, StoringParam *sp,
                           SimulationParam *prm, SimulationStatus *status, long long max_steps, double fixed_dt,
-                          int scheme)
+                          int scheme, int verbose)
 {
     const int f = system->coord_sys_flag_;
     if (f != COORD_SYS_CARTESIAN_1D && f != COORD_SYS_CYLINDRICAL_1D && f != COORD_SYS_SPHERICAL_1D) {
@@ -386,6 +408,7 @@ static ErrorStatus run_1d(BoundaryConditionParam *bc, System *system, Integrator
         const int rc =
             hfv_mh1d_run(&cfg, W, U, system->surface_area_x_, system->mid_points_x_, system->volume_, &st, NULL);
         if (rc != HFV_OK) return device_error(rc);
+        bisect_warnings(st.bisect_fails, verbose);
         if (st.err_kind != HFV_ERR_NONE || st.num_steps == cfg.steps_start) break;
         if (custom) {
             e = WRAP_TRACEBACK(set_boundary_condition(bc, system));
@@ -404,7 +427,7 @@ static ErrorStatus run_1d(BoundaryConditionParam *bc, System *system, Integrator
         cfg.steps_start = st.num_steps;
     }
     fill_status(status, &st);
-    if (st.err_kind != HFV_ERR_NONE) return run_error(st.err_kind);
+    if (st.err_kind != HFV_ERR_NONE) return run_error(st.err_kind, scheme == 2 ? 0 : verbose);
     return make_success_error_status();
 }
 
@@ -412,40 +435,35 @@ ErrorStatus hfv_muscl_hancock_2d_run(BoundaryConditionParam *bc, System *system,
                                      StoringParam *sp, Settings *settings, SimulationParam *prm,
                                      SimulationStatus *status, long long max_steps, double fixed_dt)
 {
-    (void)settings;
-    return run_2d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 0);
+    return run_2d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 0, settings ? settings->verbose : 0);
 }
 
 ErrorStatus hfv_muscl_hancock_1d_run(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
                                      StoringParam *sp, Settings *settings, SimulationParam *prm,
                                      SimulationStatus *status, long long max_steps, double fixed_dt)
 {
-    (void)settings;
-    return run_1d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 0);
+    return run_1d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 0, settings ? settings->verbose : 0);
 }
 
 ErrorStatus hfv_godunov_first_order_2d_run(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
                                            StoringParam *sp, Settings *settings, SimulationParam *prm,
                                            SimulationStatus *status, long long max_steps, double fixed_dt)
 {
-    (void)settings;
-    return run_2d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 1);
+    return run_2d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 1, settings ? settings->verbose : 0);
 }
 
 ErrorStatus hfv_godunov_first_order_1d_run(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
                                            StoringParam *sp, Settings *settings, SimulationParam *prm,
                                            SimulationStatus *status, long long max_steps, double fixed_dt)
 {
-    (void)settings;
-    return run_1d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 1);
+    return run_1d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 1, settings ? settings->verbose : 0);
 }
 
 ErrorStatus hfv_random_choice_1d_run(BoundaryConditionParam *bc, System *system, IntegratorParam *ip,
                                      StoringParam *sp, Settings *settings, SimulationParam *prm,
                                      SimulationStatus *status, long long max_steps, double fixed_dt)
 {
-    (void)settings;
-    return run_1d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 2);
+    return run_1d(bc, system, ip, sp, prm, status, max_steps, fixed_dt, 2, settings ? settings->verbose : 0);
 }
 
 ErrorStatus random_choice_1d(BoundaryConditionParam *bc, System *system, IntegratorParam *ip, StoringParam *sp,

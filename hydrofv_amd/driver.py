@@ -171,11 +171,12 @@ class Simulation:
     def launch(self, integrator, riemann_solver, slope_limiter, cfl, tf, tol=1e-6,
                shrink_factor=None, shrink_steps=None, entry="launch_simulation",
                max_steps=None, fixed_dt=None, store_dir=None, store_interval=None,
-               store_initial=False) -> RunResult:
+               store_initial=False, verbose=0) -> RunResult:
         """Run the integrator.  ``max_steps`` / ``fixed_dt`` select the
         library's step-limited / constant-dt loop (hfv_muscl_hancock_*_run,
         hfv_godunov_first_order_*_run, hfv_random_choice_1d_run).  ``store_dir`` (ending in "/") turns
-        on snapshots every ``store_interval`` of simulation time."""
+        on snapshots every ``store_interval`` of simulation time.  ``verbose``
+        is Settings.verbose (the exact solver's bisection warnings)."""
         ip = self.params(integrator, riemann_solver, slope_limiter, cfl, tol, shrink_factor, shrink_steps)
         sp = self.lib.get_new_storing_param()
         sp.is_storing = store_dir is not None
@@ -184,7 +185,7 @@ class Simulation:
             sp.output_dir = self._store_dir
             sp.storing_interval = float(store_interval)
             sp.store_initial = bool(store_initial)
-        st = abi.Settings(verbose=0, no_progress_bar=True)
+        st = abi.Settings(verbose=verbose, no_progress_bar=True)
         prm = abi.SimulationParam(tf=tf)
         status = abi.SimulationStatus()
         if max_steps is not None or fixed_dt is not None:

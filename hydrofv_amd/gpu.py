@@ -68,6 +68,7 @@ class RunStatus(C.Structure):
         ("err_cell", C.c_longlong),
         ("smax", C.c_double * 2),
         ("finished", C.c_int),
+        ("bisect_fails", C.c_longlong),
     ]
 
 

@@ -73,6 +73,11 @@ typedef struct hfv_run_status {
     long long err_cell;
     double smax[2];
     int finished;
+    /* exact-solver solves that failed after 5000 bisection iterations where
+     * the reference's loop goes on past the failure (1D MUSCL-Hancock faces,
+     * random-choice cells of the failing step); 0 on the 2D paths.  The host
+     * driver prints the reference's verbose warnings for them. */
+    long long bisect_fails;
 } hfv_run_status;
 
 typedef struct hfv_mh2d hfv_mh2d;

@@ -69,3 +69,54 @@ def test_boundaries_and_limiters(gpulib, bc, limiter):
     setup = lambda lib: K.setup_sod(lib, 50, g=3, bc=bc)
     W, U, Wo, Uo, res = run_1d(gpulib, setup, limiter, "riemann_solver_hllc", 0.8, steps=40)
     compare(f"{bc} {limiter}", W[:, None], U[:, None], Wo[:, None], Uo[:, None])
+
+
+def _bisection_warnings(text):
+    """The iteration counts of the exact solver's bisection warnings in
+    captured stderr (colour codes dropped)."""
+    import re
+    plain = re.sub(r"\x1b\[[0-9;]*m", "", text)
+    return [int(m) for m in re.findall(r"Bisection method did not converge after (\d+) iterations", plain)]
+
+
+@pytest.mark.parametrize("integrator", ["muscl_hancock_1d", "godunov_first_order_1d", "random_choice_1d"])
+def test_verbose_bisection_warnings_match_reference(gpulib, reflib, capfd, integrator):
+    """Settings.verbose on the device path: a face whose exact solve cannot
+    converge (Newton creeps for its 100 iterations, then the bisection
+    stalls at one ulp above tol = 1e-300) warns every 500 bisection
+    iterations before it fails (riemann_solver_exact.c:736-744).  One step of
+    a shock tube whose two states are such a face, run through the
+    reference build and through this library, prints the same warnings; and
+    none without verbose.  The states come from the set-aside faces of the
+    Sedov benchmark run."""
+    n, g = 64, 2
+    left = (1.00089864e+00, -1.33455157e-07, 1.07044404e-05)
+    right = (1.00181732e+00, -6.82650652e-07, 1.40474026e-05)
+    W = np.zeros((3, n + 2 * g))
+    for k in range(3):
+        W[k, :g + n // 2] = left[k]
+        W[k, g + n // 2:] = right[k]
+
+    def run(lib, verbose):
+        from hydrofv_amd import driver
+        s = driver.Simulation(lib, "cartesian_1d", 1.4, n, g=g)
+        s.set_primitive(W)
+        s.primitive_to_conserved()
+        s.set_boundary("transmissive", "transmissive")
+        capfd.readouterr()
+        try:
+            s.launch(integrator, "riemann_solver_exact", "minmod", 0.9, 1e-9, tol=1e-300, verbose=verbose)
+            failed = False
+        except driver.HydroError:
+            failed = True
+        s.close()
+        return failed, _bisection_warnings(capfd.readouterr().err)
+
+    ref_failed, ref_warn = run(reflib, 1)
+    dev_failed, dev_warn = run(gpulib, 1)
+    print(f"{integrator}: reference failed={ref_failed}, {len(ref_warn)} warnings; device failed={dev_failed}, "
+          f"{len(dev_warn)} warnings", flush=True)
+    assert len(ref_warn) >= 10 and ref_warn[:10] == list(range(500, 5001, 500))
+    assert dev_warn == ref_warn
+    assert dev_failed == ref_failed
+    assert run(gpulib, 0)[1] == []
