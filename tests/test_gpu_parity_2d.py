@@ -151,6 +151,37 @@ def test_kh_wide_strips_bitwise(gpulib):
     compare("KH 130^2 x 12", W, U, Wo, Uo)
 
 
+@pytest.mark.parametrize("solver", [2, 1], ids=["hllc_kh130", "exact_sedov96"])
+def test_strip_heights_identical(gpulib, solver):
+    """Strips of 1, 2, 3, 5 and 37 rows (shorter than the three-row window of
+    conserved rows a strip keeps in LDS for its updates, and taller than the
+    default) give the bits of the library's default height: HLLC on a shear
+    layer, the exact solver's queued faces on a blast; the default run is
+    held against the oracle by the other tests."""
+    from hydrofv_amd import problems
+    if solver == 2:
+        sim, lim, cfl, tol = K.setup_kh(gpulib, 130), 3, 0.8, 1e-6
+    else:
+        sim, lim, cfl, tol = problems.sedov(gpulib, 96), 2, 0.5, 1e-8
+    W0, U0 = sim.primitive(), sim.conserved()
+    runs = {}
+    for rows in (0, 1, 2, 3, 5, 37):
+        cfg = problems.device_config(sim, limiter=lim, solver=solver, cfl=cfl, max_steps=12, tol=tol, strip_rows=rows)
+        m = gpu.MH2D(gpulib, cfg)
+        m.upload(W0, U0)
+        m.start()
+        m.advance(11, use_graph=False)
+        st = m.status()
+        assert st.err_kind == 0 and st.num_steps == 12, (rows, st.err_kind, st.num_steps)
+        runs[rows] = (st.t, np.stack(m.download()[1]))
+        m.close()
+    sim.close()
+    t0, U0d = runs[0]
+    for rows, (t, U) in runs.items():
+        assert t == t0, rows
+        assert np.array_equal(U.view(np.uint64), U0d.view(np.uint64)), rows
+
+
 @pytest.mark.parametrize("case", ["kh130", "quadrants"])
 def test_hllc_forms_identical(gpulib, case):
     """The two HLLC code forms (selects, branches) give the same bits on the
