@@ -33,21 +33,54 @@ __global__ void __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(2)))
     strip_finish(r, out, lane);
 }
 
-// The exact-solver step with queued faces (Deferred, mh2d_strip.hpp): the
-// strip pass, then k_solve_faces, k_solve_slow and k_update_deferred.  The
-// queue counters come in three sets, one per controller slot: a step uses
-// its slot's set, and its strip pass clears the next slot's, which no
-// launch of this step reads (k_defer_reset clears a set that no step has
-// cleared: the first step of a run description).
+// The exact-solver step with queued faces (Deferred, mh2d_strip.hpp).
+//
+// Plain form (a step that reads the caller's primitive state): the strip
+// pass, then k_solve_faces, k_solve_slow and k_update_deferred.
+//
+// Split form (every other step): the faces that need the iteration lie in a
+// small share of the strip tiles (the blast region: about 4% of them in a
+// Sedov run), and the slowest of them take the reference's 100 Newton
+// iterations, a serial chain of ~100 us.  So the tiles are split in two
+// passes with queues of their own (views A and B of the queue arrays): the
+// previous step's deferred-cell update (plan_tiles) lists as A the tiles that
+// queued faces in that step and their neighbours, and the rest as B.  Pass A
+// runs first, its tiles cut in DEFER_NSUB runs of rows so that it ends
+// early, then the first pass over its faces (k_solve_faces); its set-aside
+// faces (k_solve_slow) then run on a second stream while pass B runs and
+// solves its own faces (usually none); k_update_deferred takes both views.
+// Grids of more than DEFER_PLAN_MAX_TILES tiles take the plain form.
+// Every face and cell gets the same operations on the same values in either
+// form: which pass handles a tile changes no result.
+//
+// The queue counters come in three sets per view, one per controller slot: a
+// step uses its slot's sets, and its strip pass clears the next slot's, which
+// no launch of this step reads (k_defer_reset clears the sets of a slot that
+// no step has cleared: the first step of a run description).
+constexpr int DEFER_NSUB = 6;     // row runs per tile in pass A
+constexpr int DEFER_MAX_A = 1024; // tiles in pass A at most (the rest go to B)
+
 __device__ __forceinline__ Deferred defer_slot(Deferred D, int slot)
 {
     D.count += 3 * slot;
     return D;
 }
 
+// Block 0 of a step's strip pass: publish the controller state and clear the
+// next slot's queue counters of both views (count_all: [view][slot][3]).
+__device__ __forceinline__ void step_publish(const Ctrl* in, Ctrl* out, Ctrl* clear, const StepPlan& plan,
+                                             int dt_zero, unsigned long long* count_all, int slot)
+{
+    publish(in, out, clear, plan, dt_zero);
+    const int next = 3 * ((slot + 1) % 3);
+    for (int v = 0; v < 2; v++)
+        for (int k = 0; k < 3; k++) count_all[9 * v + next + k] = 0ull;
+}
+
 template <int LIM, bool WMODE>
 __global__ void __launch_bounds__(LANES) k_mh2d_step_defer(Grid2D G, RunParams rp, Fields4 Ua, Fields4 Ub,
-                                                           Fields4 W, Ctrl* ctrl, int slot, Deferred D)
+                                                           Fields4 W, Ctrl* ctrl, int slot, Deferred D,
+                                                           unsigned long long* count_all)
 {
     const Ctrl* in = ctrl + slot;
     Ctrl* out = ctrl + (slot + 1) % 3;
@@ -55,13 +88,7 @@ __global__ void __launch_bounds__(LANES) k_mh2d_step_defer(Grid2D G, RunParams r
     int dt_zero;
     const StepPlan plan = plan_step(in, rp, &dt_zero);
     const int lane = threadIdx.x;
-    if (blockIdx.x == 0 && lane == 0) {
-        publish(in, out, clear, plan, dt_zero);
-        unsigned long long* next = D.count + 3 * ((slot + 1) % 3);
-        next[0] = 0ull;
-        next[1] = 0ull;
-        next[2] = 0ull;
-    }
+    if (blockIdx.x == 0 && lane == 0) step_publish(in, out, clear, plan, dt_zero, count_all, slot);
     if (!plan.run) return;
     const Fields4 U = plan.cur ? Ub : Ua;
     const Fields4 V = plan.cur ? Ua : Ub;
@@ -71,13 +98,98 @@ __global__ void __launch_bounds__(LANES) k_mh2d_step_defer(Grid2D G, RunParams r
     strip_finish(r, out, lane);
 }
 
-__global__ void k_defer_reset(Deferred D, int slot)
+// The split form's passes (plan: plan_tiles).  PASS 0 (A): block b takes run
+// b % DEFER_NSUB of tile A[b / DEFER_NSUB]; PASS 1 (B): block b takes tile
+// xcd_remap(b) unless it is in A, and its block 0 publishes the step.
+template <int LIM, int PASS>
+__global__ void __launch_bounds__(LANES) k_mh2d_step_split(Grid2D G, RunParams rp, Fields4 Ua, Fields4 Ub,
+                                                           Ctrl* ctrl, int slot, Deferred D,
+                                                           unsigned long long* count_all, const int* plan)
 {
-    if (threadIdx.x == 0) {
-        unsigned long long* c = D.count + 3 * slot;
-        c[0] = 0ull;
-        c[1] = 0ull;
-        c[2] = 0ull;
+    const Ctrl* in = ctrl + slot;
+    Ctrl* out = ctrl + (slot + 1) % 3;
+    Ctrl* clear = ctrl + (slot + 2) % 3;
+    int dt_zero;
+    const StepPlan sp = plan_step(in, rp, &dt_zero);
+    const int lane = threadIdx.x;
+    if (PASS == 1 && blockIdx.x == 0 && lane == 0) step_publish(in, out, clear, sp, dt_zero, count_all, slot);
+    if (!sp.run) return;
+    const int ntiles = G.nsx * G.nsy;
+    int tile, sub = 0, nsub = 1;
+    if constexpr (PASS == 0) {
+        const int pos = blockIdx.x / DEFER_NSUB;
+        if (pos >= plan[0]) return;
+        tile = plan[2 + pos];
+        if (tile < 0 || tile >= ntiles) return;
+        sub = blockIdx.x % DEFER_NSUB;
+        nsub = DEFER_NSUB;
+        int y0, y1;
+        strip_rows_of(G, tile / G.nsx, sub, nsub, &y0, &y1);
+        if (y0 >= y1) return;
+    } else {
+        tile = xcd_remap(blockIdx.x, ntiles);
+        if (plan[2 + ntiles + tile] != 0) return;
+    }
+    const Fields4 U = sp.cur ? Ub : Ua;
+    const Fields4 V = sp.cur ? Ua : Ub;
+    const Fields4 W{};  // unused: these steps read the conserved state
+    const StripResult r = strip_step<LIM, EXACT, false, false, true>(G, sp.dt, U, V, W, tile % G.nsx, tile / G.nsx,
+                                                                     lane, defer_slot(D, slot), sub, nsub);
+    strip_finish(r, out, lane);
+}
+
+// The split of the tiles for the next step, by one workgroup of T threads:
+// A holds the tiles that queued faces in this step and their eight
+// neighbours, at most cap_a of them (in no particular order); B the rest.
+// plan: [0] tiles in A, [2, 2 + ntiles) the A list, [2 + ntiles, 2 + 2 ntiles)
+// a flag per tile, 1 when the tile is in A.  The counts are read into LDS (one
+// byte per tile, up to DEFER_PLAN_MAX_TILES) and cleared for the next step.
+constexpr int DEFER_PLAN_MAX_TILES = 32768;
+
+template <int T>
+__device__ void plan_tiles(const Grid2D& G, unsigned long long* tcount, int* plan, int cap_a)
+{
+    __shared__ unsigned char had[DEFER_PLAN_MAX_TILES];
+    __shared__ int n_a;
+    const int t = threadIdx.x;
+    const int nsx = G.nsx, nsy = G.nsy, ntiles = nsx * nsy;
+    for (int k = t; k < ntiles; k += T) had[k] = tcount[k] != 0ull;
+    if (t == 0) n_a = 0;
+    __syncthreads();
+    for (int k = t; k < ntiles; k += T) tcount[k] = 0ull;
+    for (int k = t; k < ntiles; k += T) {
+        const int sx = k % nsx, sy = k / nsx;
+        bool act = false;
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int x = sx + dx, y = sy + dy;
+                if (x >= 0 && x < nsx && y >= 0 && y < nsy && had[y * nsx + x]) act = true;
+            }
+        int in_a = 0;
+        if (act) {
+            const int pos = atomicAdd(&n_a, 1);
+            if (pos < cap_a) {
+                plan[2 + pos] = k;
+                in_a = 1;
+            }
+        }
+        plan[2 + ntiles + k] = in_a;
+    }
+    __syncthreads();
+    if (t == 0) plan[0] = min(n_a, cap_a);
+}
+
+// The first step of a run description: the queue counters of both views
+// for its slot, the per-tile counts, and a plan with every tile in B.
+__global__ void k_defer_reset(unsigned long long* count_all, int slot, unsigned long long* tcount, int* plan,
+                              int tiles)
+{
+    const int t = threadIdx.x;
+    if (t < 6) count_all[9 * (t / 3) + 3 * slot + t % 3] = 0ull;
+    if (t == 0) plan[0] = 0;
+    for (int k = t; k < tiles; k += blockDim.x) {
+        tcount[k] = 0ull;
+        plan[2 + tiles + k] = 0;
     }
 }
 
@@ -184,20 +296,29 @@ __global__ void __launch_bounds__(LANES) k_solve_slow(Grid2D G, Deferred D0, Ctr
 // 213-240), and their share of the CFL maxima.
 constexpr int UPD_THREADS = 256;
 
+// views: the queue views whose deferred cells this step updates (1: the
+// plain form's, 2: both of the split form).  planner = 1: workgroup 0 plans
+// the next step's split (plan_tiles) and the others update the cells.
 __global__ void __launch_bounds__(UPD_THREADS) k_update_deferred(Grid2D G, RunParams rp, Fields4 Ua, Fields4 Ub,
-                                                                 Deferred D0, Ctrl* ctrl, int slot)
+                                                                 Deferred DA, Deferred DB, int views, Ctrl* ctrl,
+                                                                 int slot, int* tile_plan, int cap_a, int planner)
 {
+    if (planner && blockIdx.x == 0) {
+        plan_tiles<UPD_THREADS>(G, DA.tcount, tile_plan, cap_a);
+        return;
+    }
+    const long long blk = (long long)blockIdx.x - planner, nblk = (long long)gridDim.x - planner;
     int dt_zero;
     const StepPlan plan = plan_step(ctrl + slot, rp, &dt_zero);
     if (!plan.run) return;
-    const Deferred D = defer_slot(D0, slot);
     const Fields4 U = plan.cur ? Ub : Ua;
     const Fields4 V = plan.cur ? Ua : Ub;
     const double dt = plan.dt, gamma = G.gamma;
-    const long long n = (long long)min(D.count[1], (unsigned long long)D.cell_cap);
     double smx = -INFINITY, smy = -INFINITY;
-    for (long long i = (long long)blockIdx.x * UPD_THREADS + threadIdx.x; i < n;
-         i += (long long)gridDim.x * UPD_THREADS) {
+    for (int v = 0; v < views; v++) {
+    const Deferred D = defer_slot(v == 0 ? DA : DB, slot);
+    const long long n = (long long)min(D.count[1], (unsigned long long)D.cell_cap);
+    for (long long i = blk * UPD_THREADS + threadIdx.x; i < n; i += nblk * UPD_THREADS) {
         const long long e = D.cells[i];
         const int kind = (int)(e & 7);
         const long long k = e >> 3;
@@ -235,6 +356,7 @@ __global__ void __launch_bounds__(UPD_THREADS) k_update_deferred(Grid2D G, RunPa
 #pragma unroll
             for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] + D.fy[f][k - G.pitch] * dt * G.area_y;
         }
+    }
     }
     block_smax_to<UPD_THREADS>(ctrl + (slot + 1) % 3, smx, smy);
 }
@@ -445,7 +567,12 @@ struct hfv_mh2d {
     void* dev_user[4];
     long long w_filled;    // launch count at which the primitive buffer was last filled (device hooks)
     double* stage;         // one contiguous field for host copies (allocated at first use)
-    Deferred D;            // exact solver: queued faces and deferred cells
+    Deferred D;            // exact solver: queued faces and deferred cells (view A; the plain form's)
+    Deferred DB;           // exact solver: view B of the split form
+    unsigned long long* count_all;  // exact solver: both views' queue counters, [view][slot][3]
+    int* plan;             // exact solver: the split of the tiles (plan_tiles)
+    hipStream_t aux;       // exact solver: the second stream of the split form
+    hipEvent_t ev_fork, ev_join;
     long long defer_clear; // exact solver: the launch whose queue counters are known to be clear
     double* defer_slab;
 };
@@ -510,23 +637,46 @@ void launch_exact(hfv_mh2d* h, bool wmode, hipStream_t s)
 {
     const int slot = (int)(h->launches % 3);
     const Grid2D& G = h->G;
+    const int ntiles = G.nsx * G.nsy;
     const long long cells = (long long)G.tx * G.ty;
     const unsigned solve_blocks = (unsigned)std::min<long long>(4096, (2 * cells + LANES - 1) / LANES);
     const unsigned update_blocks = (unsigned)std::min<long long>(1024, (cells + UPD_THREADS - 1) / UPD_THREADS);
-    if (h->defer_clear != h->launches) hipLaunchKernelGGL(k_defer_reset, dim3(1), dim3(64), 0, s, h->D, slot);
-    const dim3 grid(G.nsx * G.nsy), block(LANES);
-    if (wmode) {
-        hipLaunchKernelGGL((k_mh2d_step_defer<LIM, true>), grid, block, 0, s, G, h->rp, h->U[0], h->U[1], h->W,
-                           h->ctrl, slot, h->D);
-    } else {
-        hipLaunchKernelGGL((k_mh2d_step_defer<LIM, false>), grid, block, 0, s, G, h->rp, h->U[0], h->U[1], h->W,
-                           h->ctrl, slot, h->D);
-    }
-    hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), dim3(LANES), 0, s, G, h->D, h->ctrl, slot);
     const unsigned slow_blocks = (unsigned)std::min<long long>(1024, (h->D.slow_cap + LANES - 1) / LANES);
-    hipLaunchKernelGGL(k_solve_slow, dim3(slow_blocks), dim3(LANES), 0, s, G, h->D, h->ctrl, slot);
-    hipLaunchKernelGGL(k_update_deferred, dim3(update_blocks), dim3(UPD_THREADS), 0, s, G, h->rp, h->U[0], h->U[1],
-                       h->D, h->ctrl, slot);
+    const int cap_a = std::min(ntiles, DEFER_MAX_A);
+    if (h->defer_clear != h->launches)
+        hipLaunchKernelGGL(k_defer_reset, dim3(1), dim3(256), 0, s, h->count_all, slot, h->D.tcount, h->plan, ntiles);
+    const dim3 block(LANES), ublock(UPD_THREADS);
+    if (wmode || ntiles > DEFER_PLAN_MAX_TILES) {
+        // plain form
+        if (wmode)
+            hipLaunchKernelGGL((k_mh2d_step_defer<LIM, true>), dim3(ntiles), block, 0, s, G, h->rp, h->U[0],
+                               h->U[1], h->W, h->ctrl, slot, h->D, h->count_all);
+        else
+            hipLaunchKernelGGL((k_mh2d_step_defer<LIM, false>), dim3(ntiles), block, 0, s, G, h->rp, h->U[0],
+                               h->U[1], h->W, h->ctrl, slot, h->D, h->count_all);
+        hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), block, 0, s, G, h->D, h->ctrl, slot);
+        hipLaunchKernelGGL(k_solve_slow, dim3(slow_blocks), block, 0, s, G, h->D, h->ctrl, slot);
+        const int planner = ntiles <= DEFER_PLAN_MAX_TILES;
+        hipLaunchKernelGGL(k_update_deferred, dim3(update_blocks + planner), ublock, 0, s, G, h->rp, h->U[0],
+                           h->U[1], h->D, h->DB, 1, h->ctrl, slot, h->plan, cap_a, planner);
+    } else {
+        // split form: pass A and its first pass on s; its set-aside faces on
+        // the second stream while pass B and its faces run on s
+        hipLaunchKernelGGL((k_mh2d_step_split<LIM, 0>), dim3(cap_a * DEFER_NSUB), block, 0, s, G, h->rp, h->U[0],
+                           h->U[1], h->ctrl, slot, h->D, h->count_all, (const int*)h->plan);
+        hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), block, 0, s, G, h->D, h->ctrl, slot);
+        (void)hipEventRecord(h->ev_fork, s);
+        (void)hipStreamWaitEvent(h->aux, h->ev_fork, 0);
+        hipLaunchKernelGGL(k_solve_slow, dim3(slow_blocks), block, 0, h->aux, G, h->D, h->ctrl, slot);
+        (void)hipEventRecord(h->ev_join, h->aux);
+        hipLaunchKernelGGL((k_mh2d_step_split<LIM, 1>), dim3(ntiles), block, 0, s, G, h->rp, h->U[0], h->U[1],
+                           h->ctrl, slot, h->DB, h->count_all, (const int*)h->plan);
+        hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), block, 0, s, G, h->DB, h->ctrl, slot);
+        hipLaunchKernelGGL(k_solve_slow, dim3(slow_blocks), block, 0, s, G, h->DB, h->ctrl, slot);
+        (void)hipStreamWaitEvent(s, h->ev_join, 0);
+        hipLaunchKernelGGL(k_update_deferred, dim3(update_blocks + 1), ublock, 0, s, G, h->rp, h->U[0], h->U[1],
+                           h->D, h->DB, 2, h->ctrl, slot, h->plan, cap_a, 1);
+    }
     h->launches++;
     h->defer_clear = h->launches;  // the strip pass cleared the next slot's counters
 }
@@ -543,7 +693,7 @@ void launch_hllc(hfv_mh2d* h, bool wmode, hipStream_t s)
 template <int LIM>
 const void* step_kernel_lim(const hfv_mh2d_config* c)
 {
-    if (c->riemann_solver == EXACT) return reinterpret_cast<const void*>(&k_mh2d_step_defer<LIM, false>);
+    if (c->riemann_solver == EXACT) return reinterpret_cast<const void*>(&k_mh2d_step_split<LIM, 1>);
     if (c->hllc_form == HFV_HLLC_BRANCHES) return reinterpret_cast<const void*>(&k_mh2d_step<LIM, HLLC, false, true>);
     return reinterpret_cast<const void*>(&k_mh2d_step<LIM, HLLC, false, false>);
 }
@@ -766,8 +916,9 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
         D.face_cap = 2 * D.cell_cap;
         // a face that cannot be set aside is finished in the first pass
         D.slow_cap = std::max<long long>(D.face_cap / 8, 64);
-        const size_t dbytes =
-            (field * 8 + (size_t)D.face_cap * 9 + (size_t)D.cell_cap + (size_t)D.slow_cap * 7 + 9) * sizeof(double);
+        const size_t ntiles = (size_t)G.nsx * G.nsy;
+        const size_t view = (size_t)D.face_cap * 9 + (size_t)D.cell_cap + (size_t)D.slow_cap * 7;
+        const size_t dbytes = (field * 8 + 2 * view + 18 + ntiles + (2 + 2 * ntiles + 1) / 2 + 1) * sizeof(double);
         if (hipMalloc(&h->defer_slab, dbytes) != hipSuccess) {
             (void)hipFree(h->slab);
             delete h;
@@ -779,20 +930,41 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
             D.fy[f] = p + field * (4 + f);
         }
         p += field * 8;
-        for (int q = 0; q < 8; q++) D.qw[q] = p + (size_t)D.face_cap * q;
-        p += (size_t)D.face_cap * 8;
-        D.qid = reinterpret_cast<long long*>(p);
-        p += D.face_cap;
-        D.cells = reinterpret_cast<long long*>(p);
-        p += D.cell_cap;
-        D.sidx = reinterpret_cast<long long*>(p);
-        p += D.slow_cap;
-        for (int q = 0; q < 5; q++) D.sst[q] = p + (size_t)D.slow_cap * q;
-        p += (size_t)D.slow_cap * 5;
-        D.sit = reinterpret_cast<int*>(p);
-        D.sbr = D.sit + D.slow_cap;
-        p += D.slow_cap;  // two int arrays in one double-sized stride
-        D.count = reinterpret_cast<unsigned long long*>(p);
+        // the queue arrays of one view
+        const auto carve = [&](Deferred& V) {
+            for (int q = 0; q < 8; q++) V.qw[q] = p + (size_t)D.face_cap * q;
+            p += (size_t)D.face_cap * 8;
+            V.qid = reinterpret_cast<long long*>(p);
+            p += D.face_cap;
+            V.cells = reinterpret_cast<long long*>(p);
+            p += D.cell_cap;
+            V.sidx = reinterpret_cast<long long*>(p);
+            p += D.slow_cap;
+            for (int q = 0; q < 5; q++) V.sst[q] = p + (size_t)D.slow_cap * q;
+            p += (size_t)D.slow_cap * 5;
+            V.sit = reinterpret_cast<int*>(p);
+            V.sbr = V.sit + D.slow_cap;
+            p += D.slow_cap;  // two int arrays in one double-sized stride
+        };
+        Deferred& DB = h->DB;
+        DB = D;
+        carve(D);
+        carve(DB);
+        h->count_all = reinterpret_cast<unsigned long long*>(p);
+        p += 18;
+        D.count = h->count_all;
+        DB.count = h->count_all + 9;
+        D.tcount = DB.tcount = reinterpret_cast<unsigned long long*>(p);
+        p += ntiles;
+        h->plan = reinterpret_cast<int*>(p);
+        if (hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
+            (void)hipFree(h->defer_slab);
+            (void)hipFree(h->slab);
+            delete h;
+            return HFV_EDEVICE;
+        }
     }
     h->launches = 0;
     h->defer_clear = -1;
@@ -800,6 +972,11 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
     h->graph_len = 0;
     h->w_filled = -1;
     if (hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess) {
+        if (h->aux) {
+            (void)hipStreamDestroy(h->aux);
+            (void)hipEventDestroy(h->ev_fork);
+            (void)hipEventDestroy(h->ev_join);
+        }
         if (h->defer_slab) (void)hipFree(h->defer_slab);
         (void)hipFree(h->slab);
         delete h;
@@ -815,6 +992,12 @@ void hfv_mh2d_destroy(hfv_mh2d* h)
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
     (void)hipStreamSynchronize(h->own);
     (void)hipStreamDestroy(h->own);
+    if (h->aux) {
+        (void)hipStreamSynchronize(h->aux);
+        (void)hipStreamDestroy(h->aux);
+        (void)hipEventDestroy(h->ev_fork);
+        (void)hipEventDestroy(h->ev_join);
+    }
     if (h->stage) (void)hipFree(h->stage);
     if (h->defer_slab) (void)hipFree(h->defer_slab);
     (void)hipFree(h->slab);

@@ -354,6 +354,9 @@ struct Deferred {
     int* sbr;
     unsigned long long* count;  // [0] queued faces, [1] deferred cells, [2] set-aside faces
     long long face_cap, cell_cap, slow_cap;
+    // faces queued per strip tile (sy * nsx + sx) by the step's strip passes,
+    // which the next step's split of the tiles reads (mh2d.hip)
+    unsigned long long* tcount;
 };
 
 enum DeferKind : int { DEF_INTERIOR = 0, DEF_GHOST_XLO = 1, DEF_GHOST_XHI = 2, DEF_GHOST_YLO = 3, DEF_GHOST_YHI = 4 };
@@ -394,11 +397,13 @@ __device__ __forceinline__ bool direct_face(double gamma, double tol, const doub
     return false;
 }
 
-// Queue the owned faces that need the iteration (states in the solver's order).
+// Queue the owned faces that need the iteration (states in the solver's order);
+// *nq (wave-uniform) counts the wavefront's queued faces.
 template <bool YFACE>
 __device__ __forceinline__ void queue_face(const Deferred& D, bool take, long long k, const double* l,
-                                           const double* r, int lane)
+                                           const double* r, int lane, unsigned long long* nq)
 {
+    *nq += (unsigned long long)__popcll(__ballot(take));
     const long long i = wave_append(&D.count[0], take, lane);
     if (i >= 0 && i < D.face_cap) {
 #pragma unroll
@@ -423,13 +428,26 @@ __device__ __forceinline__ void store_flux(double* const* F, long long k, const 
     for (int q = 0; q < 4; q++) F[q][k] = f[q];
 }
 
+// Rows [y0, y1) of strip row sy, or of part sub of it when its rows are
+// split in nsub runs of ceil(strip_rows / nsub) (empty when y0 >= y1).
+__device__ __forceinline__ void strip_rows_of(const Grid2D& G, int sy, int sub, int nsub, int* y0, int* y1)
+{
+    const int base = G.g + sy * G.strip_rows;
+    const int end = min(base + G.strip_rows, G.g + G.ny);
+    const int run = (G.strip_rows + nsub - 1) / nsub;
+    *y0 = base + sub * run;
+    *y1 = min(*y0 + run, end);
+}
+
 struct StripResult {
     double smx, smy;     // CFL maxima of the strip's updated cells
     int err;             // ErrKind of a failed flux (0: none)
     long long err_cell;
 };
 
-// One strip (sx, sy) of the step U -> V with time step dt.  BRANCHY: the
+// One strip (sx, sy) of the step U -> V with time step dt, or (nsub > 1) part
+// sub of its rows split in nsub runs of ceil(strip_rows / nsub), which the
+// caller has checked to be non-empty (strip_rows_of).  BRANCHY: the
 // HLLC faces by branches (hllc_begin_branchy) instead of selects; the same
 // bits either way.  LIM == FIRST_ORDER: the first-order Godunov step of
 // integrator_godunov_first_order.c:362-497, whose faces take the cells' own
@@ -438,7 +456,7 @@ struct StripResult {
 template <int LIM, int SOLVER, bool WMODE, bool BRANCHY, bool DEFER = false>
 __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, const Fields4& U, const Fields4& V,
                                                   const Fields4& W, int sx, int sy, int lane,
-                                                  const Deferred& D = Deferred())
+                                                  const Deferred& D = Deferred(), int sub = 0, int nsub = 1)
 {
     static_assert(!DEFER || SOLVER == EXACT, "only exact-solver faces are queued");
     const double cx = dt * G.area_x / G.volume;  // dt*A/V, integrator_muscl_hancock.c:636
@@ -448,8 +466,8 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
 
     const int g = G.g;
     const int x0 = g + sx * STRIP_COLS;        // first output column of the strip
-    const int y0 = g + sy * G.strip_rows;      // first output row
-    const int y1 = min(y0 + G.strip_rows, g + G.ny);
+    int y0, y1;                                // output rows [y0, y1)
+    strip_rows_of(G, sy, sub, nsub, &y0, &y1);
     const int col = x0 - HALO + lane;
     const bool out_col = (lane >= HALO) && (lane < HALO + STRIP_COLS) && (col < g + G.nx);
     const bool ghost_lo_col = (sx == 0) && (col == g - 1);               // x_min face flux only
@@ -463,6 +481,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     const bool need_x = out_col || ghost_hi_col || left_used;
     const bool own_x = out_col || ghost_hi_col;
     bool hyb = false;  // DEFER: the flux below the current row is queued
+    unsigned long long nq = 0ull;  // DEFER: faces this wavefront queued
 
     int err = ERR_NONE;
     long long err_cell = -1;
@@ -513,7 +532,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         }
         if constexpr (DEFER) {
             hyb = out_col && direct_face<true>(gamma, tol, wm1, wa, fyb);
-            queue_face<true>(D, hyb && y0 == g, (long long)(y0 - 1) * G.pitch + col, wm1, wa, lane);
+            queue_face<true>(D, hyb && y0 == g, (long long)(y0 - 1) * G.pitch + col, wm1, wa, lane, &nq);
         } else {
             const int e = yflux<SOLVER>(gamma, tol, wm1, wa, fyb);
             if (e && out_col) { err = e; err_cell = (long long)(y0 - 1) * G.tx + col; }
@@ -536,7 +555,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         a_row = ypredict<LIM>(wm1, wa, wb, cy, cx, gamma, lo0, yt, &cxr_row);   // row y0: bottom and top face states
         if constexpr (DEFER) {
             hyb = out_col && direct_face<true>(gamma, tol, hi_m1, lo0, fyb);
-            queue_face<true>(D, hyb && y0 == g, (long long)(y0 - 1) * G.pitch + col, hi_m1, lo0, lane);
+            queue_face<true>(D, hyb && y0 == g, (long long)(y0 - 1) * G.pitch + col, hi_m1, lo0, lane, &nq);
         } else {
             const int e = yflux<SOLVER>(gamma, tol, hi_m1, lo0, fyb);
             if (e && out_col) { err = e; err_cell = (long long)(y0 - 1) * G.tx + col; }
@@ -647,8 +666,8 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             for (int f = 0; f < 4; f++) fyt[f] = fxl[f] = 0.0;
             hy = out_col && direct_face<true>(gamma, tol, yt, ylo, fyt);
             hx = need_x && direct_face<false>(gamma, tol, hl, xlo, fxl);
-            queue_face<true>(D, hy, (long long)r * G.pitch + col, yt, ylo, lane);
-            queue_face<false>(D, hx && own_x, (long long)r * G.pitch + col, hl, xlo, lane);
+            queue_face<true>(D, hy, (long long)r * G.pitch + col, yt, ylo, lane, &nq);
+            queue_face<false>(D, hx && own_x, (long long)r * G.pitch + col, hl, xlo, lane, &nq);
             ey = ex = ERR_NONE;
         } else {
             ey = yflux<SOLVER>(gamma, tol, yt, ylo, fyt);
@@ -734,6 +753,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     if constexpr (DEFER) {
         // the top ghost row's update waits for a queued y_max face
         defer_cell(D, y1 == g + G.ny && out_col && hyb, (long long)(g + G.ny) * G.pitch + col, DEF_GHOST_YHI, lane);
+        if (lane == 0 && nq != 0ull) atomicAdd(&D.tcount[sy * G.nsx + sx], nq);
     }
     if (y1 == g + G.ny && out_col && !hyb) {
         // top ghost row receives the y_max face flux (1097-1100)
