@@ -235,6 +235,7 @@ int hfv_ens_create(int count, const hfv_mh2d_config* cfgs, hfv_ens** out)
         G.volume = c.volume;
         for (int s = 0; s < 4; s++) G.bc[s] = c.bc[s];
         G.strip_rows = c.strip_rows > 0 ? c.strip_rows : 32;
+        G.full_tiles = c.full_tiles != 0;
         G.nsx = (G.nx + STRIP_COLS - 1) / STRIP_COLS;
         G.nsy = (G.ny + G.strip_rows - 1) / G.strip_rows;
         RunParams& rp = e->host[i].rp;

@@ -1171,6 +1171,27 @@ HFV_HD __forceinline__ Sample exact_sample_resume(double gamma, double tol, doub
     return exact_sample_star(gamma, L, vl, R, vr, (r > 0) ? PStar{ps, ERR_NONE} : newton_finish(F, tol, st), speed);
 }
 
+// The Newton loop of a set-aside face from its saved state st, without the
+// finish that follows an unconverged loop: returns 1 with *P when the loop
+// converged, or when it bracketed a root it had not bracketed when it was
+// set aside (then *P is newton_finish's result); 0 when it ended without
+// converging and with the bracket as it was, so that newton_finish of the
+// saved state gives the result: newton_finish reads only the guess, f
+// there and the bracket, which the loop changes only by bracketing a root.
+HFV_HD __forceinline__ int newton_resume(const PressureFn& F, double tol, NewtonState st, PStar* P)
+{
+    const int br0 = st.bracketed;
+    double ps;
+    const int r = newton_run(F, tol, st, 100, &ps);
+    if (r > 0) {
+        *P = PStar{ps, ERR_NONE};
+        return 1;
+    }
+    if (st.bracketed == br0) return 0;
+    *P = newton_finish(F, tol, st);
+    return 1;
+}
+
 // The flux of the state sampled at x/t = 0 (exact.c:471-510; NF = 4: the
 // transverse velocity's terms added).  Returns the sample's failure kind.
 template <int NF>

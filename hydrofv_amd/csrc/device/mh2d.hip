@@ -226,9 +226,10 @@ __device__ __forceinline__ void finish_face(const Grid2D& G, const Deferred& D, 
 constexpr int NEWTON_CAP = 8;
 
 // One queued face per lane: the exact solve (riemann_solver_exact.c) of
-// exact_sample_capped.  The queue length was written by the strip pass,
-// which ran before this launch on the same stream.
-__global__ void __launch_bounds__(LANES) k_solve_faces(Grid2D G, Deferred D0, Ctrl* ctrl, int slot)
+// exact_sample_capped, with at most cap Newton iterations before a face is
+// set aside (cap 100: none is).  The queue length was written by the strip
+// pass, which ran before this launch on the same stream.
+__global__ void __launch_bounds__(LANES) k_solve_faces(Grid2D G, Deferred D0, Ctrl* ctrl, int slot, int cap)
 {
     const Deferred D = defer_slot(D0, slot);
     const long long n = (long long)min(D.count[0], (unsigned long long)D.face_cap);
@@ -243,7 +244,7 @@ __global__ void __launch_bounds__(LANES) k_solve_faces(Grid2D G, Deferred D0, Ct
         }
         NewtonState st;
         Sample S = exact_sample_capped(G.gamma, G.tol, l[0], l[1], l[2], l[3], r[0], r[1], r[2], r[3], 0.0,
-                                       NEWTON_CAP, &st);
+                                       cap, &st);
         const bool paused = S.rc == ERR_PAUSED;
         const long long j = wave_append(&D.count[2], paused, lane);
         if (paused && j < D.slow_cap) {
@@ -263,8 +264,170 @@ __global__ void __launch_bounds__(LANES) k_solve_faces(Grid2D G, Deferred D0, Ct
     }
 }
 
-// The set-aside faces, resumed from their saved iteration state.
-__global__ void __launch_bounds__(LANES) k_solve_slow(Grid2D G, Deferred D0, Ctrl* ctrl, int slot)
+// newton_finish (hfv_math.hpp; exact.c:664-760) by one wavefront, with the
+// same operations on the same values and so the same result:
+// - the bracket scan evaluates f at p_lo + j dp, j = 0..100, 64 points at a
+//   time (lane l: j = 63 r + l in round r, so that consecutive rounds share
+//   an end point), and takes the first j whose interval [j, j + 1] changes
+//   sign, the interval where the serial scan stops;
+// - the bisection evaluates, per round, the 63 midpoints of the six levels
+//   below the current interval (lane l: node l + 1 of the tree in heap
+//   order, its interval formed by the serial loop's own halvings along the
+//   node's path), then takes six serial steps that read them.
+// Every lane must call this; all return the same value.
+__device__ PStar newton_finish_wave(const PressureFn& F, double tol, const NewtonState& s, int lane)
+{
+    double p_lo = s.p_lo, f_lo = s.f_lo, p_hi = s.p_hi, f_hi = s.f_hi;
+    if (!s.bracketed) {
+        p_lo = 0.0;
+        f_lo = pressure_fn(F, p_lo);
+        if (f_lo * f_hi >= 0.0) {
+            p_hi *= 10.0;
+            const double dp = (p_hi - p_lo) / 100;
+            for (int base = 0; base < 100; base += LANES - 1) {
+                const int j = base + lane;
+                const double fj = (j <= 100) ? pressure_fn(F, p_lo + j * dp) : 0.0;
+                const double fn = __shfl_down(fj, 1, LANES);
+                const bool cross = (lane < LANES - 1) && (j < 100) && (fj * fn < 0.0);
+                const unsigned long long m = __ballot(cross);
+                if (m) {
+                    const int l = __ffsll((long long)m) - 1;
+                    const int k = base + l;
+                    f_lo = __shfl(fj, l, LANES);
+                    f_hi = __shfl(fj, l + 1, LANES);
+                    p_hi = p_lo + (k + 1) * dp;
+                    p_lo = p_lo + k * dp;
+                    break;
+                }
+            }
+            if (f_lo * f_hi >= 0.0) return PStar{0.0, ERR_EXACT_BRACKET};
+        }
+    }
+    int count = 0;
+    const int node = lane + 1;
+    const int depth = 31 - __clz(node);
+    for (;;) {
+        double nlo = p_lo, nhi = p_hi;
+        for (int d = depth - 1; d >= 0; d--) {
+            const double m = 0.5 * (nlo + nhi);
+            if ((node >> d) & 1) nlo = m;
+            else nhi = m;
+        }
+        const double fm_l = (lane < LANES - 1) ? pressure_fn(F, 0.5 * (nlo + nhi)) : 0.0;
+        int cur = 1;
+        for (int lev = 0; lev < 6; lev++) {
+            const double pm = 0.5 * (p_lo + p_hi);
+            if (p_hi - p_lo < tol) return PStar{pm, ERR_NONE};
+            const double fm = __shfl(fm_l, cur - 1, LANES);
+            if (fm * f_lo < 0.0) {
+                p_hi = pm;
+                f_hi = fm;
+                cur = 2 * cur;
+            } else {
+                p_lo = pm;
+                f_lo = fm;
+                cur = 2 * cur + 1;
+            }
+            count++;
+            if (count > 5000) return PStar{0.0, ERR_EXACT_BISECT};
+        }
+    }
+}
+
+// The bracket scan and bisection of each set-aside face, one face per
+// wavefront (Deferred::spp, sprc): newton_finish of the state the face was
+// set aside with, which k_solve_join takes for the faces whose Newton loop
+// k_solve_slow ends without converging and with the bracket unchanged.  The
+// queue length and the states were written by k_solve_faces; this launch
+// may run beside k_solve_slow's.
+__global__ void __launch_bounds__(LANES) k_solve_spec(Grid2D G, Deferred D0, int slot)
+{
+    const Deferred D = defer_slot(D0, slot);
+    const long long n = (long long)min(D.count[2], (unsigned long long)D.slow_cap);
+    if ((long long)blockIdx.x >= n) return;
+    // these waves are a latency-bound chain that runs beside the strip pass:
+    // their instructions win the SIMD's issue arbitration when ready
+    __builtin_amdgcn_s_setprio(3);
+    const int lane = threadIdx.x;
+    __shared__ double pow_lds[POW_TABS_LEN];
+    stage_pow_tables(pow_lds, lane);
+    for (long long j = blockIdx.x; j < n; j += gridDim.x) {
+        const long long i = D.sidx[j];
+        const double rl = D.qw[0][i], ul = D.qw[1][i], pl = D.qw[3][i];
+        const double rr = D.qw[4][i], ur = D.qw[5][i], pr = D.qw[7][i];
+        const double al = (rl > 0.0) ? sound_speed(G.gamma, rl, pl) : 0.0;
+        const double ar = (rr > 0.0) ? sound_speed(G.gamma, rr, pr) : 0.0;
+        const Side L{rl, ul, pl, al}, R{rr, ur, pr, ar};
+        const PressureFn F = pressure_fn_setup(G.gamma, L, R, pow_lds, pow_lds + POW_LOG_TAB_LEN);
+        // newton_finish reads the guess, f there and the bracket; the
+        // iterate and the iteration count (sst[0], sit) are k_solve_slow's,
+        // which may run at the same time and overwrite them
+        NewtonState st;
+        st.p0 = 0.0;
+        st.p_hi = D.sst[1][j];
+        st.f_hi = D.sst[2][j];
+        st.df0 = 0.0;
+        st.p_lo = D.sst[3][j];
+        st.f_lo = D.sst[4][j];
+        st.it = 0;
+        st.bracketed = D.sbr[j];
+        const PStar P = newton_finish_wave(F, G.tol, st, lane);
+        if (lane == 0) {
+            D.spp[j] = P.p;
+            D.sprc[j] = P.rc;
+        }
+    }
+}
+
+// k_solve_slow's outcome of a face whose result is newton_finish of its
+// set-aside state (k_solve_spec's)
+constexpr int SLOW_TAKE_SPEC = -100;
+
+// The set-aside faces' Newton loops, resumed from their saved state (one
+// face per lane): the outcome goes to sst[0] (p*) and sit (the failure kind,
+// or SLOW_TAKE_SPEC).  A face that brackets a root in this loop is finished
+// here, serially.
+__global__ void __launch_bounds__(LANES) k_solve_slow(Grid2D G, Deferred D0, int slot)
+{
+    const Deferred D = defer_slot(D0, slot);
+    const long long n = (long long)min(D.count[2], (unsigned long long)D.slow_cap);
+    if ((long long)blockIdx.x * LANES >= n) return;
+    // these waves are a latency-bound chain that runs beside the strip pass:
+    // their instructions win the SIMD's issue arbitration when ready
+    __builtin_amdgcn_s_setprio(3);
+    __shared__ double pow_lds[POW_TABS_LEN];
+    stage_pow_tables(pow_lds, threadIdx.x);
+    for (long long j = (long long)blockIdx.x * LANES + threadIdx.x; j < n; j += (long long)gridDim.x * LANES) {
+        const long long i = D.sidx[j];
+        const double rl = D.qw[0][i], ul = D.qw[1][i], pl = D.qw[3][i];
+        const double rr = D.qw[4][i], ur = D.qw[5][i], pr = D.qw[7][i];
+        const double al = (rl > 0.0) ? sound_speed(G.gamma, rl, pl) : 0.0;
+        const double ar = (rr > 0.0) ? sound_speed(G.gamma, rr, pr) : 0.0;
+        const Side L{rl, ul, pl, al}, R{rr, ur, pr, ar};
+        const PressureFn F = pressure_fn_setup(G.gamma, L, R, pow_lds, pow_lds + POW_LOG_TAB_LEN);
+        NewtonState st;
+        st.p0 = D.sst[0][j];
+        st.p_hi = D.sst[1][j];
+        st.f_hi = D.sst[2][j];
+        st.df0 = 0.0;  // used by iteration 0 only
+        st.p_lo = D.sst[3][j];
+        st.f_lo = D.sst[4][j];
+        st.it = D.sit[j];
+        st.bracketed = D.sbr[j];
+        PStar P;
+        if (newton_resume(F, G.tol, st, &P)) {
+            D.sst[0][j] = P.p;
+            D.sit[j] = P.rc;
+        } else {
+            D.sit[j] = SLOW_TAKE_SPEC;
+        }
+    }
+}
+
+// The set-aside faces' samples and fluxes, from k_solve_slow's outcome or
+// k_solve_spec's finish (both complete before this launch), one face per
+// lane.
+__global__ void __launch_bounds__(LANES) k_solve_join(Grid2D G, Deferred D0, Ctrl* ctrl, int slot)
 {
     const Deferred D = defer_slot(D0, slot);
     const long long n = (long long)min(D.count[2], (unsigned long long)D.slow_cap);
@@ -277,17 +440,12 @@ __global__ void __launch_bounds__(LANES) k_solve_slow(Grid2D G, Deferred D0, Ctr
             l[q] = D.qw[q][i];
             r[q] = D.qw[4 + q][i];
         }
-        NewtonState st;
-        st.p0 = D.sst[0][j];
-        st.p_hi = D.sst[1][j];
-        st.f_hi = D.sst[2][j];
-        st.df0 = 0.0;  // used by iteration 0 only
-        st.p_lo = D.sst[3][j];
-        st.f_lo = D.sst[4][j];
-        st.it = D.sit[j];
-        st.bracketed = D.sbr[j];
-        const Sample S = exact_sample_resume(G.gamma, G.tol, l[0], l[1], l[2], l[3], r[0], r[1], r[2], r[3], 0.0, st);
-        finish_face(G, D, i, S, out);
+        const int code = D.sit[j];
+        const PStar P = (code == SLOW_TAKE_SPEC) ? PStar{D.spp[j], D.sprc[j]} : PStar{D.sst[0][j], code};
+        const double al = (l[0] > 0.0) ? sound_speed(G.gamma, l[0], l[3]) : 0.0;
+        const double ar = (r[0] > 0.0) ? sound_speed(G.gamma, r[0], r[3]) : 0.0;
+        const Side L{l[0], l[1], l[3], al}, R{r[0], r[1], r[3], ar};
+        finish_face(G, D, i, exact_sample_star(G.gamma, L, l[2], R, r[2], P, 0.0), out);
     }
 }
 
@@ -532,6 +690,47 @@ __global__ void k_copy_fields(Grid2D G, Fields4 dst, Fields4 src, const Ctrl* ct
     for (int f = 0; f < 4; f++) dst.f[f][k] = src.f[f][k];
 }
 
+// Probe of the pressure iteration's finish (hfv_probe_exact_finish): the
+// Newton loop from the guess for cap iterations, then, for a face it has not
+// converged, newton_finish (WAVE 0: one face per lane) or newton_finish_wave
+// (WAVE 1: one face per wavefront) from the state it left.  rc: the failure
+// kind, or -1 for a face whose loop converged (p: the converged iterate), -3
+// for a vacuum face (no iteration).
+template <bool WAVE>
+__global__ void __launch_bounds__(LANES) k_probe_finish(double gamma, double tol, int cap, long long n,
+                                                        const double* wl, const double* wr, double* p, int* rc)
+{
+    const int lane = threadIdx.x;
+    const long long i = WAVE ? (long long)blockIdx.x : (long long)blockIdx.x * LANES + lane;
+    if (i >= n) return;
+    const double rl = wl[4 * i], ul = wl[4 * i + 1], pl = wl[4 * i + 3];
+    const double rr = wr[4 * i], ur = wr[4 * i + 1], pr = wr[4 * i + 3];
+    const double al = (rl > 0.0) ? sound_speed(gamma, rl, pl) : 0.0;
+    const double ar = (rr > 0.0) ? sound_speed(gamma, rr, pr) : 0.0;
+    double out_p = 0.0;
+    int out_rc = -3;
+    if (!(rl <= 0.0 || rr <= 0.0 || ((al + ar) * 2.0 / (gamma + 1.0)) <= (ur - ul))) {
+        const Side L{rl, ul, pl, al}, R{rr, ur, pr, ar};
+        const PressureFn F = pressure_fn_setup(gamma, L, R);
+        NewtonState st = newton_begin(gamma, L, R, tol, F);
+        double ps;
+        if (newton_run(F, tol, st, cap, &ps) > 0) {
+            out_p = ps;
+            out_rc = -1;
+        } else {
+            PStar P;
+            if constexpr (WAVE) P = newton_finish_wave(F, tol, st, lane);
+            else P = newton_finish(F, tol, st);
+            out_p = P.p;
+            out_rc = P.rc;
+        }
+    }
+    if (!WAVE || lane == 0) {
+        p[i] = out_p;
+        rc[i] = out_rc;
+    }
+}
+
 // Working memory of hfv_mh2d_run_noalloc on each GPU: the three controller
 // slots in its first 256 bytes, then the second conserved buffer.  Host code
 // takes its address per device with hipGetSymbolAddress (scratch_base) and
@@ -572,7 +771,8 @@ struct hfv_mh2d {
     unsigned long long* count_all;  // exact solver: both views' queue counters, [view][slot][3]
     int* plan;             // exact solver: the split of the tiles (plan_tiles)
     hipStream_t aux;       // exact solver: the second stream of the split form
-    hipEvent_t ev_fork, ev_join;
+    hipStream_t aux2;      // exact solver: its third stream
+    hipEvent_t ev_fork, ev_join, ev_join2;
     long long defer_clear; // exact solver: the launch whose queue counters are known to be clear
     double* defer_slab;
 };
@@ -642,6 +842,7 @@ void launch_exact(hfv_mh2d* h, bool wmode, hipStream_t s)
     const unsigned solve_blocks = (unsigned)std::min<long long>(4096, (2 * cells + LANES - 1) / LANES);
     const unsigned update_blocks = (unsigned)std::min<long long>(1024, (cells + UPD_THREADS - 1) / UPD_THREADS);
     const unsigned slow_blocks = (unsigned)std::min<long long>(1024, (h->D.slow_cap + LANES - 1) / LANES);
+    const unsigned spec_blocks = (unsigned)std::min<long long>(2048, h->D.slow_cap);
     const int cap_a = std::min(ntiles, DEFER_MAX_A);
     if (h->defer_clear != h->launches)
         hipLaunchKernelGGL(k_defer_reset, dim3(1), dim3(256), 0, s, h->count_all, slot, h->D.tcount, h->plan, ntiles);
@@ -654,25 +855,34 @@ void launch_exact(hfv_mh2d* h, bool wmode, hipStream_t s)
         else
             hipLaunchKernelGGL((k_mh2d_step_defer<LIM, false>), dim3(ntiles), block, 0, s, G, h->rp, h->U[0],
                                h->U[1], h->W, h->ctrl, slot, h->D, h->count_all);
-        hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), block, 0, s, G, h->D, h->ctrl, slot);
-        hipLaunchKernelGGL(k_solve_slow, dim3(slow_blocks), block, 0, s, G, h->D, h->ctrl, slot);
+        hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), block, 0, s, G, h->D, h->ctrl, slot, NEWTON_CAP);
+        hipLaunchKernelGGL(k_solve_spec, dim3(spec_blocks), block, 0, s, G, h->D, slot);
+        hipLaunchKernelGGL(k_solve_slow, dim3(slow_blocks), block, 0, s, G, h->D, slot);
+        hipLaunchKernelGGL(k_solve_join, dim3(slow_blocks), block, 0, s, G, h->D, h->ctrl, slot);
         const int planner = ntiles <= DEFER_PLAN_MAX_TILES;
         hipLaunchKernelGGL(k_update_deferred, dim3(update_blocks + planner), ublock, 0, s, G, h->rp, h->U[0],
                            h->U[1], h->D, h->DB, 1, h->ctrl, slot, h->plan, cap_a, planner);
     } else {
-        // split form: pass A and its first pass on s; its set-aside faces on
-        // the second stream while pass B and its faces run on s
+        // split form: pass A and its first face pass on s; its set-aside
+        // faces' Newton loops on aux and their speculative finish on aux2,
+        // joined on aux, while pass B runs on s and solves its own faces
+        // (rarely any: a tile with none in the previous step and none next to
+        // it) in their first pass
         hipLaunchKernelGGL((k_mh2d_step_split<LIM, 0>), dim3(cap_a * DEFER_NSUB), block, 0, s, G, h->rp, h->U[0],
                            h->U[1], h->ctrl, slot, h->D, h->count_all, (const int*)h->plan);
-        hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), block, 0, s, G, h->D, h->ctrl, slot);
+        hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), block, 0, s, G, h->D, h->ctrl, slot, NEWTON_CAP);
         (void)hipEventRecord(h->ev_fork, s);
         (void)hipStreamWaitEvent(h->aux, h->ev_fork, 0);
-        hipLaunchKernelGGL(k_solve_slow, dim3(slow_blocks), block, 0, h->aux, G, h->D, h->ctrl, slot);
+        (void)hipStreamWaitEvent(h->aux2, h->ev_fork, 0);
+        hipLaunchKernelGGL(k_solve_slow, dim3(slow_blocks), block, 0, h->aux, G, h->D, slot);
+        hipLaunchKernelGGL(k_solve_spec, dim3(spec_blocks), block, 0, h->aux2, G, h->D, slot);
+        (void)hipEventRecord(h->ev_join2, h->aux2);
+        (void)hipStreamWaitEvent(h->aux, h->ev_join2, 0);
+        hipLaunchKernelGGL(k_solve_join, dim3(slow_blocks), block, 0, h->aux, G, h->D, h->ctrl, slot);
         (void)hipEventRecord(h->ev_join, h->aux);
         hipLaunchKernelGGL((k_mh2d_step_split<LIM, 1>), dim3(ntiles), block, 0, s, G, h->rp, h->U[0], h->U[1],
                            h->ctrl, slot, h->DB, h->count_all, (const int*)h->plan);
-        hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), block, 0, s, G, h->DB, h->ctrl, slot);
-        hipLaunchKernelGGL(k_solve_slow, dim3(slow_blocks), block, 0, s, G, h->DB, h->ctrl, slot);
+        hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), block, 0, s, G, h->DB, h->ctrl, slot, 100);
         (void)hipStreamWaitEvent(s, h->ev_join, 0);
         hipLaunchKernelGGL(k_update_deferred, dim3(update_blocks + 1), ublock, 0, s, G, h->rp, h->U[0], h->U[1],
                            h->D, h->DB, 2, h->ctrl, slot, h->plan, cap_a, 1);
@@ -831,6 +1041,7 @@ void grid_from_config(const hfv_mh2d_config* cfg, long long pitch, Grid2D& G, Ru
     G.nsx = (G.nx + STRIP_COLS - 1) / STRIP_COLS;
     G.strip_rows = cfg->strip_rows > 0 ? cfg->strip_rows : balanced_strip_rows(G.nx, G.ny, resident_step_waves(cfg));
     G.nsy = (G.ny + G.strip_rows - 1) / G.strip_rows;
+    G.full_tiles = cfg->full_tiles != 0;
     rp = RunParams();
     rp.cfl = cfg->cfl;
     rp.shrink_factor = cfg->cfl_initial_shrink_factor;
@@ -917,7 +1128,7 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
         // a face that cannot be set aside is finished in the first pass
         D.slow_cap = std::max<long long>(D.face_cap / 8, 64);
         const size_t ntiles = (size_t)G.nsx * G.nsy;
-        const size_t view = (size_t)D.face_cap * 9 + (size_t)D.cell_cap + (size_t)D.slow_cap * 7;
+        const size_t view = (size_t)D.face_cap * 9 + (size_t)D.cell_cap + (size_t)D.slow_cap * 9;
         const size_t dbytes = (field * 8 + 2 * view + 18 + ntiles + (2 + 2 * ntiles + 1) / 2 + 1) * sizeof(double);
         if (hipMalloc(&h->defer_slab, dbytes) != hipSuccess) {
             (void)hipFree(h->slab);
@@ -945,6 +1156,10 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
             V.sit = reinterpret_cast<int*>(p);
             V.sbr = V.sit + D.slow_cap;
             p += D.slow_cap;  // two int arrays in one double-sized stride
+            V.spp = p;
+            p += D.slow_cap;
+            V.sprc = reinterpret_cast<int*>(p);
+            p += D.slow_cap;  // one int array in a double-sized stride
         };
         Deferred& DB = h->DB;
         DB = D;
@@ -958,8 +1173,16 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
         p += ntiles;
         h->plan = reinterpret_cast<int*>(p);
         if (hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&h->aux2, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_join2, hipEventDisableTiming) != hipSuccess) {
+            // the handle was value-initialised: what was not created is null
+            if (h->aux) (void)hipStreamDestroy(h->aux);
+            if (h->aux2) (void)hipStreamDestroy(h->aux2);
+            if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+            if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+            if (h->ev_join2) (void)hipEventDestroy(h->ev_join2);
             (void)hipFree(h->defer_slab);
             (void)hipFree(h->slab);
             delete h;
@@ -974,8 +1197,10 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
     if (hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess) {
         if (h->aux) {
             (void)hipStreamDestroy(h->aux);
+            (void)hipStreamDestroy(h->aux2);
             (void)hipEventDestroy(h->ev_fork);
             (void)hipEventDestroy(h->ev_join);
+            (void)hipEventDestroy(h->ev_join2);
         }
         if (h->defer_slab) (void)hipFree(h->defer_slab);
         (void)hipFree(h->slab);
@@ -994,9 +1219,12 @@ void hfv_mh2d_destroy(hfv_mh2d* h)
     (void)hipStreamDestroy(h->own);
     if (h->aux) {
         (void)hipStreamSynchronize(h->aux);
+        (void)hipStreamSynchronize(h->aux2);
         (void)hipStreamDestroy(h->aux);
+        (void)hipStreamDestroy(h->aux2);
         (void)hipEventDestroy(h->ev_fork);
         (void)hipEventDestroy(h->ev_join);
+        (void)hipEventDestroy(h->ev_join2);
     }
     if (h->stage) (void)hipFree(h->stage);
     if (h->defer_slab) (void)hipFree(h->defer_slab);
@@ -1176,6 +1404,39 @@ int hfv_mh2d_set_device_bc(hfv_mh2d* h, int side, hfv_device_bc_2d_func fn, void
     h->dev_user[side] = fn ? user : nullptr;
     h->w_filled = -1;
     return HFV_OK;
+}
+
+int hfv_probe_exact_finish(int impl, double gamma, double tol, int cap, long long n, const double* wl,
+                           const double* wr, double* p, int* rc)
+{
+    if ((impl != 0 && impl != 1) || cap < 0 || cap > 100 || n < 0) return HFV_EINVAL;
+    if (n == 0) return HFV_OK;
+    double *dl = nullptr, *dr = nullptr, *dp = nullptr;
+    int* dc = nullptr;
+    int ret = HFV_OK;
+    if (hipMalloc(&dl, (size_t)n * 4 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dr, (size_t)n * 4 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dp, (size_t)n * sizeof(double)) != hipSuccess || hipMalloc(&dc, (size_t)n * sizeof(int)) != hipSuccess) {
+        ret = HFV_ENOMEM;
+    } else if (hipMemcpy(dl, wl, (size_t)n * 4 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(dr, wr, (size_t)n * 4 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        ret = HFV_EDEVICE;
+    } else {
+        if (impl == 1)
+            hipLaunchKernelGGL(k_probe_finish<true>, dim3((unsigned)n), dim3(LANES), 0, 0, gamma, tol, cap, n, dl, dr,
+                               dp, dc);
+        else
+            hipLaunchKernelGGL(k_probe_finish<false>, dim3((unsigned)((n + LANES - 1) / LANES)), dim3(LANES), 0, 0,
+                               gamma, tol, cap, n, dl, dr, dp, dc);
+        if (hipGetLastError() != hipSuccess || hipMemcpy(p, dp, (size_t)n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(rc, dc, (size_t)n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+            ret = HFV_EDEVICE;
+    }
+    (void)hipFree(dl);
+    (void)hipFree(dr);
+    (void)hipFree(dp);
+    (void)hipFree(dc);
+    return ret;
 }
 
 const char* hfv_device_error_string(void)

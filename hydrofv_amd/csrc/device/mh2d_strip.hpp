@@ -50,6 +50,7 @@ struct Grid2D {
     int bc[4];           // x_min, x_max, y_min, y_max
     int strip_rows;
     int nsx, nsy;        // strips in x and y
+    int full_tiles = 0;  // 1: no uniform-tile form (tile_uniform)
 };
 
 // Boundary mapping of a ghost column (row) index onto the interior cell whose
@@ -352,6 +353,11 @@ struct Deferred {
     double* sst[5];
     int* sit;
     int* sbr;
+    // per set-aside face: the bracket scan and bisection that finish its
+    // iteration if the Newton loop ends without converging and without a new
+    // bracket (k_solve_spec): p* and the failure kind
+    double* spp;
+    int* sprc;
     unsigned long long* count;  // [0] queued faces, [1] deferred cells, [2] set-aside faces
     long long face_cap, cell_cap, slow_cap;
     // faces queued per strip tile (sy * nsx + sx) by the step's strip passes,
@@ -445,6 +451,51 @@ struct StripResult {
     long long err_cell;
 };
 
+// True when every cell that the rows [y0, y1) of a strip read -- rows
+// y0 - AHEAD .. y1 + AHEAD - 1 of the wavefront's 64 columns, as load_raw
+// maps them -- holds the same conserved state as lane 0's first one, bit for
+// bit, with the same velocity flips.  Then every cell and face of those rows
+// performs the same operations on the same values, so each row's results
+// are those of the first row.  Rows are loaded 8 at a time (the last row
+// repeated past the end), and the test stops at the first batch that holds
+// a difference: on a tile without a uniform region it costs about one row's
+// loads, which the step reads again from L2.
+template <int AHEAD>
+__device__ __forceinline__ bool tile_uniform(const Grid2D& G, const Fields4& U, int y0, int y1, int col)
+{
+    const Fields4 none{};
+    RawCell ref;
+    load_raw<false>(G, U, none, y0 - AHEAD, col, ref);
+    unsigned long long rb[4];
+    bool same = true;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(ref.v[f]);
+        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+        const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+        rb[f] = ((unsigned long long)hi << 32) | lo;
+        same &= b == rb[f];
+    }
+    const int fu0 = __builtin_amdgcn_readfirstlane(ref.fu), fv0 = __builtin_amdgcn_readfirstlane(ref.fv);
+    same &= (ref.fu == fu0) & (ref.fv == fv0);
+    if (!wave_all(same)) return false;
+    constexpr int BATCH = 8;
+    const int last = y1 + AHEAD - 1;
+    for (int q = y0 - AHEAD + 1; q <= last; q += BATCH) {
+        RawCell c[BATCH];
+#pragma unroll
+        for (int j = 0; j < BATCH; j++) load_raw<false>(G, U, none, min(q + j, last), col, c[j]);
+#pragma unroll
+        for (int j = 0; j < BATCH; j++) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) same &= (unsigned long long)__double_as_longlong(c[j].v[f]) == rb[f];
+            same &= (c[j].fu == fu0) & (c[j].fv == fv0);
+        }
+        if (!wave_all(same)) return false;
+    }
+    return true;
+}
+
 // One strip (sx, sy) of the step U -> V with time step dt, or (nsub > 1) part
 // sub of its rows split in nsub runs of ceil(strip_rows / nsub), which the
 // caller has checked to be non-empty (strip_rows_of).  BRANCHY: the
@@ -490,6 +541,11 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     __shared__ double pow_lds[POW_TABS_LEN];
     if constexpr (SOLVER == HLLC) stage_pow_tables(pow_lds, lane);
     constexpr int AHEAD = (LIM == FIRST_ORDER) ? 1 : 2;  // the row a step's iteration r reads
+    // A tile of one state computes its first row and stores that row's
+    // results in its other rows (tile_uniform); the first step of a run, which
+    // reads the primitive state, does not take this form.
+    const bool uniform = !WMODE && !G.full_tiles && tile_uniform<AHEAD>(G, U, y0, y1, col);
+    int rep_row = -1;  // a uniform tile: the row whose results the rows above it take
     // The conserved states of the last AHEAD + 1 rows loaded, each lane its
     // own column (slot q % (AHEAD + 1) holds row q): a row's update takes its
     // conserved state from here, as loaded for its c2p AHEAD iterations
@@ -701,6 +757,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             }
             defer_cell(D, deferred, k, kind, lane);
         }
+        double u[4];  // the updated state of an updated cell
         if (deferred) {
             // updated by k_update_deferred
         } else if (out_col) {
@@ -709,7 +766,6 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
 #pragma unroll
                 for (int f = 0; f < 4; f++) uk[f] = us[f * LANES];
             }
-            double u[4];
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 double x = uk[f];
@@ -749,6 +805,41 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         a_row = a_next;
         cxr_row = cxr_next;
         hyb = hy;
+
+        // A uniform tile whose first row queued, deferred and failed nothing:
+        // its other rows receive that row's results below, and the top face
+        // flux of its last row is fyb as it stands.  The lane's result (the
+        // updated state, or a ghost column's face flux) waits in urows' slot
+        // of row r, which no later read needs.
+        if (uniform && wave_all(!deferred && !hx && !hy && err == ERR_NONE)) {
+            double* rs = urows + (r % (AHEAD + 1)) * 4 * LANES + lane;
+#pragma unroll
+            for (int f = 0; f < 4; f++) rs[f * LANES] = out_col ? u[f] : ghost_lo_col ? fxr[f] : fxl[f];
+            rep_row = r;
+            break;
+        }
+    }
+    if (rep_row >= 0) {
+        // the other rows of a uniform tile: the same operations on the same
+        // values as row rep_row, except that a ghost column's update takes
+        // the ghost cell's own stored state
+        const double* rs = urows + (rep_row % (AHEAD + 1)) * 4 * LANES + lane;
+        double v[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++) v[f] = rs[f * LANES];
+        for (int q = rep_row + 1; q < y1; q++) {
+            const long long kq = (long long)q * G.pitch + col;
+            if (out_col) {
+#pragma unroll
+                for (int f = 0; f < 4; f++) V.f[f][kq] = v[f];
+            } else if (ghost_lo_col) {
+#pragma unroll
+                for (int f = 0; f < 4; f++) V.f[f][kq] = U.f[f][kq] - v[f] * dt * G.area_x;
+            } else if (ghost_hi_col) {
+#pragma unroll
+                for (int f = 0; f < 4; f++) V.f[f][kq] = U.f[f][kq] + v[f] * dt * G.area_x;
+            }
+        }
     }
     if constexpr (DEFER) {
         // the top ghost row's update waits for a queued y_max face

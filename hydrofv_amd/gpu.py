@@ -33,6 +33,7 @@ class MH2DConfig(C.Structure):
         ("strip_rows", C.c_int),
         ("hllc_form", C.c_int),
         ("first_order", C.c_int),
+        ("full_tiles", C.c_int),
     ]
 
 
@@ -95,6 +96,8 @@ PROTOTYPES = {
     "hfv_probe_pow": (C.c_int, [C.c_int, C.c_longlong, dp, dp, dp]),
     "hfv_probe_sqrt": (C.c_int, [C.c_int, C.c_longlong, dp, dp]),
     "hfv_probe_div": (C.c_int, [C.c_int, C.c_longlong, dp, dp, dp]),
+    "hfv_probe_exact_finish": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, C.c_longlong, dp, dp, dp,
+                                         _P(C.c_int)]),
     "hfv_device_error_string": (C.c_char_p, []),
     "hfv_muscl_hancock_1d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
     "hfv_muscl_hancock_2d_run": (abi.ErrorStatus, list(_SEVEN) + [C.c_longlong, C.c_double]),
@@ -176,6 +179,21 @@ def probe_div(lib, impl: int, a: np.ndarray, b: np.ndarray):
     out = np.zeros_like(a)
     check(lib, lib.hfv_probe_div(impl, a.size, _ptr(a), _ptr(b), _ptr(out)), "hfv_probe_div")
     return out
+
+
+def probe_exact_finish(lib, impl: int, gamma: float, tol: float, cap: int, wl: np.ndarray, wr: np.ndarray):
+    """The exact solver's pressure iteration: Newton for cap iterations, then
+    the finish of a face not converged, serially (impl 0) or by a wavefront
+    per face (impl 1).  Returns (p*, rc); rc -1: converged within cap, -3:
+    vacuum, else the failure kind."""
+    wl = np.ascontiguousarray(wl, dtype=np.float64)
+    wr = np.ascontiguousarray(wr, dtype=np.float64)
+    n = wl.shape[0]
+    p = np.zeros(n)
+    rc = np.zeros(n, dtype=np.int32)
+    check(lib, lib.hfv_probe_exact_finish(impl, gamma, tol, cap, n, _ptr(wl), _ptr(wr), _ptr(p),
+                                          rc.ctypes.data_as(C.POINTER(C.c_int))), "hfv_probe_exact_finish")
+    return p, rc
 
 
 class MH2D:

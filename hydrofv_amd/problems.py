@@ -46,7 +46,7 @@ def four_quadrant(lib, n: int, states, g: int = 2) -> driver.Simulation:
 
 def device_config(sim: driver.Simulation, limiter: int, solver: int, cfl: float, tf: float = float("inf"),
                   max_steps: int = -1, fixed_dt: float = 0.0, tol: float = 1e-6, strip_rows: int = 0,
-                  hllc_form: int = 0):
+                  hllc_form: int = 0, full_tiles: int = 0):
     """hfv_mh2d_config for a 2D Simulation (flags as in the reference headers)."""
     from . import gpu
     from .abi import BOUNDARY_CONDITION_NONE
@@ -56,7 +56,7 @@ def device_config(sim: driver.Simulation, limiter: int, solver: int, cfl: float,
         gravity=sy.gravity, dx=sy.dx_, dy=sy.dy_, area_x=sy.surface_area_x_[0], area_y=sy.surface_area_y_[0],
         volume=sy.volume_[0], slope_limiter=limiter, riemann_solver=solver, tol=tol, cfl=cfl,
         cfl_initial_shrink_factor=0.2, cfl_initial_shrink_num_steps=10, tf=tf, max_steps=max_steps,
-        fixed_dt=fixed_dt, strip_rows=strip_rows, hllc_form=hllc_form)
+        fixed_dt=fixed_dt, strip_rows=strip_rows, hllc_form=hllc_form, full_tiles=full_tiles)
     flags = [sim.bc.boundary_condition_flag_x_min_, sim.bc.boundary_condition_flag_x_max_,
              sim.bc.boundary_condition_flag_y_min_, sim.bc.boundary_condition_flag_y_max_]
     for k, f in enumerate(flags):

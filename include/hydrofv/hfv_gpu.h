@@ -55,6 +55,9 @@ typedef struct hfv_mh2d_config {
                                       integrator_godunov_first_order.c:270-570): no
                                       reconstruction, slope_limiter ignored, 1 ghost
                                       layer suffices; 0: MUSCL-Hancock */
+    int full_tiles;                /* 1: every strip computes each of its rows; 0 (default): a strip
+                                      whose stencil holds one state bit for bit computes one row
+                                      and stores its results in the others (the same bits) */
 } hfv_mh2d_config;
 
 /* Two code forms of the HLLC flux with identical results: SELECTS evaluates
@@ -287,6 +290,16 @@ int hfv_probe_sqrt(int impl, long long n, const double *x, double *out);
 int hfv_probe_div(int impl, long long n, const double *a, const double *b, double *out);
 /* out[n] = limit_slope(a[n], b[n]) for limiter 1/2/3 (slope_limiter.c:93-115). */
 int hfv_probe_limit(int limiter, long long n, const double *a, const double *b, double *out);
+
+/* The exact solver's pressure iteration (riemann_solver_exact.c:604-763) for the left and right
+ * states wl[n][4], wr[n][4] (rho, u, v, p): Newton from the guess for cap (0-100) iterations,
+ * then, for a face not converged, its finish (bracket scan and bisection) as the kernels take
+ * it: impl 0 serially per face, impl 1 by a whole wavefront per face (the set-aside faces of
+ * the 2D step).  p[n] = p*, rc[n] = the failure kind (HFV_ERR_*), -1 for a face that
+ * converged within cap iterations (p: its iterate), -3 for a vacuum face.  impl 1 must equal
+ * impl 0 bit for bit. */
+int hfv_probe_exact_finish(int impl, double gamma, double tol, int cap, long long n, const double *wl,
+                           const double *wr, double *p, int *rc);
 
 /* Text of the most recent HIP runtime error (for error messages). */
 const char *hfv_device_error_string(void);

@@ -48,6 +48,26 @@ def setup_random_2d(lib, nx, ny, g, bcs, seed, gamma=1.4, gravity=0.0):
     return s
 
 
+def setup_blob_2d(lib, nx, ny, g, bcs, gamma=1.4, gravity=0.0, flow=(1.0, 0.4, -0.3, 1.0), radius=0.08):
+    """A uniform flow `flow` (rho, u, v, p) with a denser, hotter disc at the
+    centre of [0,1]^2: away from the disc every cell holds the same state."""
+    s = driver.Simulation(lib, "cartesian_2d", gamma, nx, ny, g, gravity=gravity)
+    s.set_boundary(*bcs)
+    # every cell, ghost cells included, when a side has no boundary fill
+    X, Y = np.meshgrid(s.mid_x, s.mid_y)
+    W = [np.full(X.shape, flow[k]) for k in range(4)]
+    disc = (X - 0.5) ** 2 + (Y - 0.5) ** 2 < radius ** 2
+    W[0] = np.where(disc, 2.0 * flow[0], W[0])
+    W[3] = np.where(disc, 3.0 * flow[3], W[3])
+    if "none" in bcs:
+        s.set_primitive(W)
+    else:
+        s.set_primitive([w[g:g + ny, g:g + nx] for w in W], interior_only=True)
+    s.apply_boundary()
+    s.primitive_to_conserved()
+    return s
+
+
 from hydrofv_amd import problems  # noqa: E402
 
 setup_kh = problems.kelvin_helmholtz

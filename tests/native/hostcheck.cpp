@@ -3,6 +3,7 @@
 // that CPU tests can check the kernels' own source: forward pieces against
 // the oracle, derivative pieces against automatic differentiation.
 // Test infrastructure only; built by tests/native/Makefile.
+#include <cstring>
 #include "../../hydrofv_amd/csrc/device/hfv_adjoint.hpp"
 #include "../../hydrofv_amd/csrc/device/hfv_pow.hpp"
 #include "../../hydrofv_amd/csrc/device/hfv_strips.hpp"
@@ -67,6 +68,41 @@ void hc_exact_sample_capped(int cap, double gamma, double tol, long long n, cons
     }
 }
 
+// The set-aside faces' split solve, face by face: Newton to cap iterations;
+// for a face still iterating, newton_resume from that state, and where it
+// leaves the result to the finish, newton_finish of the set-aside state (the
+// 2D step's k_solve_slow, k_solve_spec and k_solve_join).  same[n] = 1 when
+// p* and the failure kind equal solve_p_star's bit for bit; took_spec[n] = 1
+// when the finish of the set-aside state gave the result.
+void hc_resume_split(int cap, double gamma, double tol, long long n, const double* wl, const double* wr, int* same,
+                     int* took_spec)
+{
+    for (long long i = 0; i < n; i++) {
+        const double* l = wl + 4 * i;
+        const double* r = wr + 4 * i;
+        same[i] = 1;
+        took_spec[i] = 0;
+        if (!(l[0] > 0.0) || !(r[0] > 0.0)) continue;
+        const double al = sound_speed(gamma, l[0], l[3]), ar = sound_speed(gamma, r[0], r[3]);
+        if (((al + ar) * 2.0 / (gamma + 1.0)) <= (r[1] - l[1])) continue;
+        const Side L{l[0], l[1], l[3], al}, R{r[0], r[1], r[3], ar};
+        const PStar full = solve_p_star(gamma, L, R, tol);
+        const PressureFn F = pressure_fn_setup(gamma, L, R);
+        NewtonState st = newton_begin(gamma, L, R, tol, F);
+        double ps;
+        const int rc = newton_run(F, tol, st, cap, &ps);
+        PStar P;
+        if (rc > 0) P = PStar{ps, ERR_NONE};
+        else if (rc < 0) P = newton_finish(F, tol, st);
+        else if (!newton_resume(F, tol, st, &P)) {
+            P = newton_finish(F, tol, st);
+            took_spec[i] = 1;
+        }
+        const bool eq = (P.rc == full.rc) && (P.rc != ERR_NONE || memcmp(&P.p, &full.p, sizeof(double)) == 0);
+        same[i] = eq ? 1 : 0;
+    }
+}
+
 // sizes and selected field offsets of the structs of hfv_gpu.h, for the
 // ctypes mirrors: out[0..3] sizeof(mh2d, mh1d, run_status, adj2d config),
 // out[4..] offsets of the fields added last to each config
@@ -76,7 +112,7 @@ void hc_struct_layout(long long* out)
     out[1] = sizeof(hfv_mh1d_config);
     out[2] = sizeof(hfv_run_status);
     out[3] = sizeof(hfv_adj2d_config);
-    out[4] = offsetof(hfv_mh2d_config, first_order);
+    out[4] = offsetof(hfv_mh2d_config, full_tiles);
     out[5] = offsetof(hfv_mh1d_config, t_pause);
     out[6] = offsetof(hfv_mh1d_config, random_choice);
     out[7] = offsetof(hfv_adj2d_config, slope_limiter);

@@ -163,7 +163,7 @@ def test_device_config_mirrors_match_the_header():
     L.hc_struct_layout(out)
     assert list(out[:4]) == [C.sizeof(gpu.MH2DConfig), C.sizeof(gpu.MH1DConfig), C.sizeof(gpu.RunStatus),
                              C.sizeof(adjoint.Adj2DConfig)]
-    assert out[4] == gpu.MH2DConfig.first_order.offset
+    assert out[4] == gpu.MH2DConfig.full_tiles.offset
     assert out[5] == gpu.MH1DConfig.t_pause.offset
     assert out[6] == gpu.MH1DConfig.random_choice.offset
     assert out[7] == adjoint.Adj2DConfig.slope_limiter.offset

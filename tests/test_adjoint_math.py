@@ -446,6 +446,33 @@ def test_exact_solver_capped_and_resumed_is_bitwise(hc):
         assert (rc0 != 0).sum() >= 50  # the NaN faces fail (bisection)
 
 
+def test_exact_solver_split_resume_is_bitwise(hc):
+    """The 2D step's split of a set-aside face: its Newton loop resumed
+    (newton_resume) and, where that loop ends unconverged with the bracket
+    unchanged, newton_finish of the state the face was set aside with (the
+    finish the device evaluates beside the loop) give solve_p_star's p* and
+    failure kind bit for bit; blast-like quiet-zone states (pressures near
+    1e-5, whose loop creeps for its 100 iterations) take that finish."""
+    hc.hc_resume_split.argtypes = [C.c_int, C.c_double, C.c_double, C.c_longlong, _dp, _dp, _ip, _ip]
+    r = np.random.default_rng(29)
+    n = 20_000
+    rho = 10 ** r.uniform(-2, 1, (n, 2)); p = 10 ** r.uniform(-6, 5, (n, 2)); u = r.uniform(-30, 30, (n, 2))
+    q = np.stack([1.0 + 0.01 * r.uniform(size=(n, 2)), 1e-6 * r.uniform(-1, 1, (n, 2)),
+                  1e-5 * (1 + 10 * r.uniform(size=(n, 2)))])
+    rho, u, p = np.concatenate([rho, q[0]]), np.concatenate([u, q[1]]), np.concatenate([p, q[2]])
+    p[:50, 0] = np.nan
+    m = rho.shape[0]
+    v = r.normal(size=(m, 2))
+    wl = np.ascontiguousarray(np.stack([rho[:, 0], u[:, 0], v[:, 0], p[:, 0]], axis=1))
+    wr = np.ascontiguousarray(np.stack([rho[:, 1], u[:, 1], v[:, 1], p[:, 1]], axis=1))
+    for tol in (1e-6, 1e-8):
+        for cap in (0, 8, 50):
+            same, took = np.empty(m, dtype=np.int32), np.empty(m, dtype=np.int32)
+            hc.hc_resume_split(cap, 1.4, tol, m, wl, wr, same, took)
+            assert same.all(), (tol, cap, np.flatnonzero(same == 0)[:5])
+            assert took.sum() > 300, (tol, cap, took.sum())
+
+
 def _expansions(n, seed, gamma, spread):
     """Two-rarefaction faces with pressures within `spread` of each other."""
     r = np.random.default_rng(seed)

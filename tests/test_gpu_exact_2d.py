@@ -107,3 +107,41 @@ def test_sedov_taylor_shock_radius(gpulib):
     R = 1.004 * (1.0 * 0.05 ** 2 / 1.0) ** 0.25
     print(f"Sedov 256^2: density peak at r = {r_peak:.4f}, Sedov-Taylor R = {R:.4f} ({(r_peak - R) / R:+.2%})")
     assert abs(r_peak - R) / R < 0.04
+
+
+def _finish_states():
+    """Face states for the pressure iteration's finish: the branch-covering
+    pairs, and pairs like a blast's quiet zone (pressures near 1e-5, tiny
+    velocities), whose Newton loop with the reference's derivative creeps
+    for its 100 iterations and ends in the bracket scan and bisection."""
+    from hydrofv_amd import workloads
+    wl, wr = K.riemann_states(3000, 29)
+    n = 3000
+    u = workloads.uniform01(31, 6 * n).reshape(6, n)
+    ql = np.stack([1.0 + 0.01 * u[0], 1e-6 * (2 * u[1] - 1), 1e-4 * (2 * u[2] - 1), 1e-5 * (1 + 10 * u[3])], axis=1)
+    qr = np.stack([1.0 + 0.01 * u[4], 1e-6 * (2 * u[5] - 1), 1e-4 * (2 * u[2] - 1), 1e-5 * (1 + 10 * u[1])], axis=1)
+    return np.concatenate([wl, ql]), np.concatenate([wr, qr])
+
+
+@pytest.mark.parametrize("tol", [1e-8, 1e-6, 0.0])
+@pytest.mark.parametrize("cap", [0, 8, 100])
+def test_wave_finish_is_the_serial_finish(gpulib, cap, tol):
+    """The set-aside faces' bracket scan and bisection by a whole wavefront
+    (k_solve_spec's newton_finish_wave) against the serial newton_finish,
+    from the state Newton leaves after cap iterations: the same p* and
+    failure kind bit for bit, over states that end in the scan's bracket, in
+    its failure, in bisection, and (tol 0) in the bisection's 5000-step
+    failure."""
+    from hydrofv_amd import gpu
+    wl, wr = _finish_states()
+    p0, rc0 = gpu.probe_exact_finish(gpulib, 0, 1.4, tol, cap, wl, wr)
+    p1, rc1 = gpu.probe_exact_finish(gpulib, 1, 1.4, tol, cap, wl, wr)
+    assert np.array_equal(rc0, rc1), np.flatnonzero(rc0 != rc1)[:10]
+    same = (p0.view(np.uint64) == p1.view(np.uint64))
+    assert same.all(), [(i, p0[i], p1[i], rc0[i]) for i in np.flatnonzero(~same)[:5]]
+    finished = rc0 >= 0
+    print(f"cap {cap} tol {tol}: finished {finished.sum()}, bracket failures {(rc0 == 3).sum()}, "
+          f"bisection failures {(rc0 == 4).sum()}, converged {(rc0 == -1).sum()}, vacuum {(rc0 == -3).sum()}")
+    assert finished.sum() > 100
+    if tol == 0.0:
+        assert (rc0 == 4).sum() > 100

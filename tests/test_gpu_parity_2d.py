@@ -208,6 +208,84 @@ def test_hllc_forms_identical(gpulib, case):
     assert np.array_equal(out[0][1], out[1][1])
 
 
+def far_from_disc(sim, r=0.3):
+    """Cells (ghost cells included) farther than r from the centre of [0,1]^2."""
+    X, Y = np.meshgrid(sim.mid_x, sim.mid_y)
+    return (X - 0.5) ** 2 + (Y - 0.5) ** 2 > r * r
+
+
+@pytest.mark.parametrize("bcs", ["periodic", "transmissive", "reflective", "mixed", "none"], ids=str)
+@pytest.mark.parametrize("limiter", K.LIMITERS)
+def test_uniform_regions_bitwise(gpulib, bcs, limiter):
+    """A uniform flow (rho 1, u 0.4, v -0.3, p 1) around a denser disc, 190 x 150
+    cells: the strip tiles away from the disc hold one state, so each computes
+    one row and gives its results to its other rows (ghost columns and rows
+    included); the disc's tiles take the full path.  Within the tolerance
+    everywhere (the disc's two-rarefaction faces take pow), bitwise away from
+    the disc, where every face has equal states."""
+    setup = lambda lib: K.setup_blob_2d(lib, 190, 150, 2, K.BC_SETS[bcs])
+    sim, W, U, Wo, Uo = run_pair(gpulib, setup, limiter, steps=12)
+    compare(f"disc {bcs} {limiter}", W, U, Wo, Uo)
+    compare(f"disc {bcs} {limiter}, far field", W, U, Wo, Uo, bitwise=True, mask=far_from_disc(sim))
+    sim.close()
+
+
+@pytest.mark.parametrize("case", ["sedov96_exact", "quadrants150_hllc_selects", "quadrants150_hllc_branches",
+                                  "disc_gravity_first_order"])
+def test_uniform_tiles_identical(gpulib, case):
+    """The uniform-tile form (the default) against every row computed
+    (full_tiles = 1), 40 steps: the same bits and the same time, on a blast
+    in a quiescent medium (the exact solver's split passes and queued
+    faces), a four-quadrant problem (both HLLC forms) and a disc in a
+    uniform flow with gravity (first-order Godunov)."""
+    from hydrofv_amd import problems, workloads
+    kw = {}
+    if case.startswith("sedov"):
+        sim, kw = problems.sedov(gpulib, 96), dict(limiter=2, solver=1, cfl=0.5, tol=1e-8)
+    elif case.startswith("quadrants"):
+        sim = problems.four_quadrant(gpulib, 150, workloads.ensemble_specs(1, seed=3)[1][0])
+        kw = dict(limiter=1, solver=2, cfl=0.8,
+                  hllc_form=gpu.HLLC_BRANCHES if case.endswith("branches") else gpu.HLLC_SELECTS)
+    else:
+        sim = K.setup_blob_2d(gpulib, 130, 70, 1, K.BC_SETS["reflective"], gravity=0.7)
+        kw = dict(limiter=1, solver=2, cfl=0.8)
+    W0, U0 = sim.primitive(), sim.conserved()
+    out = []
+    for full in (0, 1):
+        cfg = problems.device_config(sim, max_steps=40, full_tiles=full, **kw)
+        if case.startswith("disc"):
+            cfg.first_order = 1
+        m = gpu.MH2D(gpulib, cfg)
+        m.upload(W0, U0)
+        m.start()
+        m.advance(39, use_graph=False)
+        st = m.status()
+        assert st.err_kind == 0 and st.num_steps == 40, (full, st.err_kind, st.num_steps)
+        W, U = m.download()
+        out.append((st.t, np.stack(W), np.stack(U)))
+        m.close()
+    sim.close()
+    assert out[0][0] == out[1][0]
+    for k in (1, 2):
+        assert np.array_equal(out[0][k].view(np.uint64), out[1][k].view(np.uint64)), k
+
+
+@pytest.mark.parametrize("integrator", ["muscl_hancock_2d", "godunov_first_order_2d"])
+def test_uniform_regions_gravity_and_first_order(gpulib, integrator):
+    """The same with gravity 0.7 and reflective walls (the source term on the
+    replicated rows), for both 2D integrators."""
+    sim = K.setup_blob_2d(gpulib, 130, 70, 2, K.BC_SETS["reflective"], gravity=0.7)
+    grid = K.oracle_grid(sim, "van_leer")
+    res = sim.launch(integrator, "riemann_solver_hllc", "van_leer", 0.8, tf=np.inf, max_steps=11)
+    r = grid.run(0.8, max_steps=11, first_order=integrator.startswith("godunov"))
+    assert r.error == 0 and res.num_steps == r.num_steps == 11
+    assert abs(res.t - r.t) <= TOL * r.t
+    W, U = K.state(sim)
+    compare(f"disc gravity {integrator}", W, U, grid.W, grid.U)
+    compare(f"disc gravity {integrator}, far field", W, U, grid.W, grid.U, bitwise=True, mask=far_from_disc(sim))
+    sim.close()
+
+
 def test_gravity_bitwise(gpulib):
     setup = lambda lib: K.setup_random_2d(lib, 13, 9, 2, K.BC_SETS["reflective"], 11, gravity=0.7)
     sim, W, U, Wo, Uo = run_pair(gpulib, setup, "van_leer", steps=11)
