@@ -147,7 +147,7 @@ def autotune(ens: Ensemble, sides, stream, trial_launches=48, log=None):
     import torch
 
     sh = stream.cuda_stream
-    rules = [("fixed", v) for v in (16, 32, 64)] + [("prop", d) for d in (4, 8, 16, 32)]
+    rules = [("fixed", v) for v in (16, 24, 32, 48, 64)] + [("prop", d) for d in (4, 8, 16, 32, 48, 64)]
     forms = (gpu.HLLC_SELECTS, gpu.HLLC_BRANCHES) if ens.cfgs[0].riemann_solver == 2 else (gpu.HLLC_SELECTS,)
     best = None
     results = []
