@@ -309,6 +309,61 @@ __device__ __forceinline__ void tsrs_compact(double gamma, const HllcFace (&F)[2
     }
 }
 
+// The HLLC fluxes of a lane's two faces in one row: the top face of its
+// cell (states yt below, ylo above, (rho, u, v, p) order) and its left face
+// (hl from the left neighbour, xlo its own); fyt and fxl in (mass, mx, my,
+// E) order.  The wavefront's two-shock guesses and two-rarefaction powers
+// are taken together (tsrs_compact, trrs_powers), so every lane must call
+// this.  trrs_q: 6 * 64 doubles of LDS; pow_lds: the staged pow tables.
+template <bool BRANCHY>
+__device__ __forceinline__ void hllc_row_faces(double gamma, double tol, const double* yt, const double* ylo,
+                                               const double* hl, const double* xlo, double* fyt, double* fxl,
+                                               int* ey, int* ex, double* trrs_q, int lane, const double* pow_lds)
+{
+    HllcFace F[2];  // y face (normal velocity first), x face
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        const int t = (f == 1) ? 2 : (f == 2) ? 1 : f;
+        F[0].wl[f] = yt[t];
+        F[0].wr[f] = ylo[t];
+        F[1].wl[f] = hl[f];
+        F[1].wr[f] = xlo[f];
+    }
+    double o[2][4];
+    if constexpr (BRANCHY) {
+        hllc_begin_branchy(gamma, tol, F[0]);
+        hllc_begin_branchy(gamma, tol, F[1]);
+        trrs_powers(gamma, tol, F[0], F[1], trrs_q, lane, pow_lds);
+        *ey = hllc_end_branchy(gamma, F[0], o[0]);
+        *ex = hllc_end_branchy(gamma, F[1], o[1]);
+    } else {
+        HllcGuess<2> Gs;
+        hllc_guess_begin<2>(gamma, F, Gs);
+        tsrs_compact(gamma, F, Gs, trrs_q, lane);
+        hllc_guess_end<2>(tol, F, Gs);
+        trrs_powers(gamma, tol, F[0], F[1], trrs_q, lane, pow_lds);
+        // both faces' regular fluxes together, then the rare vacuum faces
+        int e2[2];
+        hllc_end_regular_faces<2>(gamma, F, o, e2);
+        *ey = e2[0];
+        *ex = e2[1];
+        if (F[0].kind == HLLC_VACUUM) {
+            hllc_vacuum_flux<4>(gamma, F[0].wl, F[0].al, F[0].wr, F[0].ar, o[0]);
+            *ey = ERR_NONE;
+        }
+        if (F[1].kind == HLLC_VACUUM) {
+            hllc_vacuum_flux<4>(gamma, F[1].wl, F[1].al, F[1].wr, F[1].ar, o[1]);
+            *ex = ERR_NONE;
+        }
+    }
+    fyt[0] = o[0][0];
+    fyt[1] = o[0][2];
+    fyt[2] = o[0][1];
+    fyt[3] = o[0][3];
+#pragma unroll
+    for (int f = 0; f < 4; f++) fxl[f] = o[1][f];
+}
+
 // Neighbour exchange across the wavefront by the GFX9 whole-wavefront DPP
 // shifts (v_mov_b32_dpp wave_shr:1 / wave_shl:1), register to register, two
 // 32-bit moves per double: lane i receives lane i-1's (from_left) or lane
@@ -454,17 +509,18 @@ struct StripResult {
 // True when every cell that the rows [y0, y1) of a strip read -- rows
 // y0 - AHEAD .. y1 + AHEAD - 1 of the wavefront's 64 columns, as load_raw
 // maps them -- holds the same conserved state as lane 0's first one, bit for
-// bit, with the same velocity flips.  Then every cell and face of those rows
-// performs the same operations on the same values, so each row's results
-// are those of the first row.  Rows are loaded 8 at a time (the last row
-// repeated past the end), and the test stops at the first batch that holds
-// a difference: on a tile without a uniform region it costs about one row's
-// loads, which the step reads again from L2.
+// bit, with the same velocity flips; ref: the lane's first cell as loaded
+// (that state, when true).  Then every cell and face of those rows performs
+// the same operations on the same values (uniform_strip).  The first row is
+// tested alone, then rows 16 at a time (the last row repeated past the end),
+// and the test stops at the first batch that holds a difference: on a tile
+// without a uniform region it costs one row's loads, which the step reads
+// again from L2.
 template <int AHEAD>
-__device__ __forceinline__ bool tile_uniform(const Grid2D& G, const Fields4& U, int y0, int y1, int col)
+__device__ __forceinline__ bool tile_uniform(const Grid2D& G, const Fields4& U, int y0, int y1, int col,
+                                             RawCell& ref)
 {
     const Fields4 none{};
-    RawCell ref;
     load_raw<false>(G, U, none, y0 - AHEAD, col, ref);
     unsigned long long rb[4];
     bool same = true;
@@ -479,7 +535,7 @@ __device__ __forceinline__ bool tile_uniform(const Grid2D& G, const Fields4& U, 
     const int fu0 = __builtin_amdgcn_readfirstlane(ref.fu), fv0 = __builtin_amdgcn_readfirstlane(ref.fv);
     same &= (ref.fu == fu0) & (ref.fv == fv0);
     if (!wave_all(same)) return false;
-    constexpr int BATCH = 8;
+    constexpr int BATCH = 16;
     const int last = y1 + AHEAD - 1;
     for (int q = y0 - AHEAD + 1; q <= last; q += BATCH) {
         RawCell c[BATCH];
@@ -492,6 +548,105 @@ __device__ __forceinline__ bool tile_uniform(const Grid2D& G, const Fields4& U, 
             same &= (c[j].fu == fu0) & (c[j].fv == fv0);
         }
         if (!wave_all(same)) return false;
+    }
+    return true;
+}
+
+// A uniform tile (tile_uniform): every cell and face of rows [y0, y1)
+// performs the operations of one cell on the same values, so the results
+// are computed once, from the tile's state rc, by the calls the row loop of
+// strip_step makes, in its order, with every neighbour equal to the cell
+// itself (a neighbour exchange then returns the lane's own value, and every
+// slope is limit(0, 0)), and stored in every row: the interior cells' new
+// state, the ghost columns' one-face updates (each from its own stored
+// state) and the ghost rows' one-face updates at the grid's bottom and top.
+// Returns false, having written nothing, when a face would be queued (DEFER)
+// or fails: the caller then runs the rows as usual.  *smx, *smy: the updated
+// cells' CFL maxima.  Every lane must call this (wavefront-wide guesses and
+// powers in HLLC).
+template <int LIM, int SOLVER, bool BRANCHY, bool DEFER>
+__device__ __forceinline__ bool uniform_strip(const Grid2D& G, double dt, const Fields4& U, const Fields4& V,
+                                              const RawCell& rc, int y0, int y1, int col, bool out_col,
+                                              bool ghost_lo_col, bool ghost_hi_col, double cx, double cy,
+                                              double rvol, bool vol_ok, double* trrs_q, int lane,
+                                              const double* pow_lds, double* smx, double* smy)
+{
+    const double gamma = G.gamma, tol = G.tol;
+    double w[4];
+    raw_to_prim<false>(G, rc, w, rvol, vol_ok);
+    double ylo[4], yhi[4], xlo[4], xhi[4];
+    if constexpr (LIM == FIRST_ORDER) {
+#pragma unroll
+        for (int f = 0; f < 4; f++) ylo[f] = yhi[f] = xlo[f] = xhi[f] = w[f];
+    } else {
+        double cxr;
+        const double a = ypredict<LIM>(w, w, w, cy, cx, gamma, ylo, yhi, &cxr);
+        double d[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++) d[f] = limit<LIM>(w[f] - w[f], w[f] - w[f]);
+        predict_x_cr(w, d, cx, cxr, a, xlo, xhi);
+    }
+    // a row's top face (yhi below, ylo above) and a lane's left face (the
+    // left neighbour's xhi, its own xlo); the bottom and right faces have the
+    // same states
+    double fy[4], fx[4];
+    int ey, ex;
+    if constexpr (SOLVER == HLLC) {
+        hllc_row_faces<BRANCHY>(gamma, tol, yhi, ylo, xhi, xlo, fy, fx, &ey, &ex, trrs_q, lane, pow_lds);
+    } else if constexpr (DEFER) {
+        const bool qy = direct_face<true>(gamma, tol, yhi, ylo, fy);
+        const bool qx = direct_face<false>(gamma, tol, xhi, xlo, fx);
+        if (!wave_all(!qy && !qx)) return false;
+        ey = ex = ERR_NONE;
+    } else {
+        ey = yflux<SOLVER>(gamma, tol, yhi, ylo, fy);
+        ex = flux<SOLVER, 4>(gamma, tol, xhi, xlo, fx);
+    }
+    if (!wave_all((ey == ERR_NONE) & (ex == ERR_NONE))) return false;
+    // an interior cell's update (integrator_muscl_hancock.c:806-814,
+    // 1092-1100; source_term.c:213-240) and its CFL maxima (utils.c:75-92)
+    double u[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        double x = rc.v[f];
+        x += fx[f] * dt * G.area_x;
+        x -= fx[f] * dt * G.area_x;
+        x += fy[f] * dt * G.area_y;
+        x -= fy[f] * dt * G.area_y;
+        u[f] = x;
+    }
+    if (G.gravity != 0.0) gravity_source(dt, G.gravity, u[0], &u[2], &u[3]);
+    if (out_col) {
+        double wn[4];
+        c2p_2d_rn(gamma, G.volume, rvol, vol_ok, u[0], u[1], u[2], u[3], wn);
+        const double a = sound_speed_rn(gamma, wn[0], wn[3]);
+        const double sx_ = a + fabs(wn[1]);
+        const double sy_ = a + fabs(wn[2]);
+        if (sx_ > *smx) *smx = sx_;
+        if (sy_ > *smy) *smy = sy_;
+    }
+    for (int q = y0; q < y1; q++) {
+        const long long k = (long long)q * G.pitch + col;
+        if (out_col) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) V.f[f][k] = u[f];
+        } else if (ghost_lo_col) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] - fx[f] * dt * G.area_x;
+        } else if (ghost_hi_col) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] + fx[f] * dt * G.area_x;
+        }
+    }
+    if (y0 == G.g && out_col) {
+        const long long k = (long long)(G.g - 1) * G.pitch + col;
+#pragma unroll
+        for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] - fy[f] * dt * G.area_y;
+    }
+    if (y1 == G.g + G.ny && out_col) {
+        const long long k = (long long)(G.g + G.ny) * G.pitch + col;
+#pragma unroll
+        for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] + fy[f] * dt * G.area_y;
     }
     return true;
 }
@@ -541,11 +696,23 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     __shared__ double pow_lds[POW_TABS_LEN];
     if constexpr (SOLVER == HLLC) stage_pow_tables(pow_lds, lane);
     constexpr int AHEAD = (LIM == FIRST_ORDER) ? 1 : 2;  // the row a step's iteration r reads
-    // A tile of one state computes its first row and stores that row's
-    // results in its other rows (tile_uniform); the first step of a run, which
-    // reads the primitive state, does not take this form.
-    const bool uniform = !WMODE && !G.full_tiles && tile_uniform<AHEAD>(G, U, y0, y1, col);
-    int rep_row = -1;  // a uniform tile: the row whose results the rows above it take
+    // A tile of one state takes the results of one cell (tile_uniform,
+    // uniform_strip); the first step of a run, which reads the primitive
+    // state, does not take this form.
+    if constexpr (!WMODE) {
+        RawCell uref;
+        if (!G.full_tiles && tile_uniform<AHEAD>(G, U, y0, y1, col, uref) &&
+            uniform_strip<LIM, SOLVER, BRANCHY, DEFER>(G, dt, U, V, uref, y0, y1, col, out_col, ghost_lo_col,
+                                                        ghost_hi_col, cx, cy, rvol, vol_ok, trrs_q, lane, pow_lds,
+                                                        &smx, &smy)) {
+            StripResult res;
+            res.smx = smx;
+            res.smy = smy;
+            res.err = ERR_NONE;
+            res.err_cell = -1;
+            return res;
+        }
+    }
     // The conserved states of the last AHEAD + 1 rows loaded, each lane its
     // own column (slot q % (AHEAD + 1) holds row q): a row's update takes its
     // conserved state from here, as loaded for its c2p AHEAD iterations
@@ -673,50 +840,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         int ey, ex;
         bool hy = false, hx = false;  // DEFER: this row's top face / the lane's x face is queued
         if constexpr (SOLVER == HLLC) {
-            // both faces' fluxes, with the wavefront's two-rarefaction
-            // powers taken together
-            HllcFace F[2];  // y face (normal velocity first), x face
-#pragma unroll
-            for (int f = 0; f < 4; f++) {
-                const int t = (f == 1) ? 2 : (f == 2) ? 1 : f;
-                F[0].wl[f] = yt[t];
-                F[0].wr[f] = ylo[t];
-                F[1].wl[f] = hl[f];
-                F[1].wr[f] = xlo[f];
-            }
-            double o[2][4];
-            if constexpr (BRANCHY) {
-                hllc_begin_branchy(gamma, tol, F[0]);
-                hllc_begin_branchy(gamma, tol, F[1]);
-                trrs_powers(gamma, tol, F[0], F[1], trrs_q, lane, pow_lds);
-                ey = hllc_end_branchy(gamma, F[0], o[0]);
-                ex = hllc_end_branchy(gamma, F[1], o[1]);
-            } else {
-                HllcGuess<2> Gs;
-                hllc_guess_begin<2>(gamma, F, Gs);
-                tsrs_compact(gamma, F, Gs, trrs_q, lane);
-                hllc_guess_end<2>(tol, F, Gs);
-                trrs_powers(gamma, tol, F[0], F[1], trrs_q, lane, pow_lds);
-                // both faces' regular fluxes together, then the rare vacuum faces
-                int e2[2];
-                hllc_end_regular_faces<2>(gamma, F, o, e2);
-                ey = e2[0];
-                ex = e2[1];
-                if (F[0].kind == HLLC_VACUUM) {
-                    hllc_vacuum_flux<4>(gamma, F[0].wl, F[0].al, F[0].wr, F[0].ar, o[0]);
-                    ey = ERR_NONE;
-                }
-                if (F[1].kind == HLLC_VACUUM) {
-                    hllc_vacuum_flux<4>(gamma, F[1].wl, F[1].al, F[1].wr, F[1].ar, o[1]);
-                    ex = ERR_NONE;
-                }
-            }
-            fyt[0] = o[0][0];
-            fyt[1] = o[0][2];
-            fyt[2] = o[0][1];
-            fyt[3] = o[0][3];
-#pragma unroll
-            for (int f = 0; f < 4; f++) fxl[f] = o[1][f];
+            hllc_row_faces<BRANCHY>(gamma, tol, yt, ylo, hl, xlo, fyt, fxl, &ey, &ex, trrs_q, lane, pow_lds);
         } else if constexpr (DEFER) {
 #pragma unroll
             for (int f = 0; f < 4; f++) fyt[f] = fxl[f] = 0.0;
@@ -805,41 +929,6 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
         a_row = a_next;
         cxr_row = cxr_next;
         hyb = hy;
-
-        // A uniform tile whose first row queued, deferred and failed nothing:
-        // its other rows receive that row's results below, and the top face
-        // flux of its last row is fyb as it stands.  The lane's result (the
-        // updated state, or a ghost column's face flux) waits in urows' slot
-        // of row r, which no later read needs.
-        if (uniform && wave_all(!deferred && !hx && !hy && err == ERR_NONE)) {
-            double* rs = urows + (r % (AHEAD + 1)) * 4 * LANES + lane;
-#pragma unroll
-            for (int f = 0; f < 4; f++) rs[f * LANES] = out_col ? u[f] : ghost_lo_col ? fxr[f] : fxl[f];
-            rep_row = r;
-            break;
-        }
-    }
-    if (rep_row >= 0) {
-        // the other rows of a uniform tile: the same operations on the same
-        // values as row rep_row, except that a ghost column's update takes
-        // the ghost cell's own stored state
-        const double* rs = urows + (rep_row % (AHEAD + 1)) * 4 * LANES + lane;
-        double v[4];
-#pragma unroll
-        for (int f = 0; f < 4; f++) v[f] = rs[f * LANES];
-        for (int q = rep_row + 1; q < y1; q++) {
-            const long long kq = (long long)q * G.pitch + col;
-            if (out_col) {
-#pragma unroll
-                for (int f = 0; f < 4; f++) V.f[f][kq] = v[f];
-            } else if (ghost_lo_col) {
-#pragma unroll
-                for (int f = 0; f < 4; f++) V.f[f][kq] = U.f[f][kq] - v[f] * dt * G.area_x;
-            } else if (ghost_hi_col) {
-#pragma unroll
-                for (int f = 0; f < 4; f++) V.f[f][kq] = U.f[f][kq] + v[f] * dt * G.area_x;
-            }
-        }
     }
     if constexpr (DEFER) {
         // the top ghost row's update waits for a queued y_max face

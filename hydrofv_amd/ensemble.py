@@ -134,6 +134,10 @@ def strip_rule(sides, kind, value):
     return np.clip(np.rint(sides / value), 8, 512).astype(int)
 
 
+# the strip-height rules the autotuner times (strip_rule)
+STRIP_RULES = [("fixed", v) for v in (16, 24, 32, 48, 64)] + [("prop", d) for d in (4, 8, 16, 32, 48, 64)]
+
+
 def autotune(ens: Ensemble, sides, stream, trial_launches=48, log=None):
     """Pick the strip heights and the HLLC code form (hfv_mh2d_config.hllc_form)
     that minimise the ensemble step time.  Every candidate starts from the
@@ -147,7 +151,7 @@ def autotune(ens: Ensemble, sides, stream, trial_launches=48, log=None):
     import torch
 
     sh = stream.cuda_stream
-    rules = [("fixed", v) for v in (16, 24, 32, 48, 64)] + [("prop", d) for d in (4, 8, 16, 32, 48, 64)]
+    rules = STRIP_RULES
     forms = (gpu.HLLC_SELECTS, gpu.HLLC_BRANCHES) if ens.cfgs[0].riemann_solver == 2 else (gpu.HLLC_SELECTS,)
     best = None
     results = []

@@ -111,7 +111,7 @@ def test_autotune_picks_a_timed_candidate(gpulib, small_batch):
     sides, cfgs, init = small_batch
     ens = ensemble.Ensemble(gpulib, cfgs, init)
     best, results = ensemble.autotune(ens, sides, torch.cuda.Stream(), trial_launches=2)
-    assert len(results) == 14 and {r[0] for r in results} == {0, 1}
+    assert len(results) == 2 * len(ensemble.STRIP_RULES) and {r[0] for r in results} == {0, 1}
     assert best[3] == min(r[3] for r in results)
     ens.start()
     ens.advance(24, use_graph=False)
