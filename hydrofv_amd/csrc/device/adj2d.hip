@@ -326,7 +326,10 @@ __global__ void __launch_bounds__(LANES) k_fwd_step(Grid2D G, double dt, Fields4
 {
     const int ntiles = G.nsx * G.nsy;
     const int b = xcd_remap(blockIdx.x, ntiles);
-    const StripResult r = strip_step<LIM, HLLC, false, false>(G, dt, U, V, U, b % G.nsx, b / G.nsx, threadIdx.x);
+    // the adjoint workload's states are smooth fields without tiles of one
+    // state: the step is built without that form (the same results)
+    const StripResult r =
+        strip_step<LIM, HLLC, false, false, false, false>(G, dt, U, V, U, b % G.nsx, b / G.nsx, threadIdx.x);
     if (__ballot(r.err != ERR_NONE) && threadIdx.x == 0) atomicOr(flags, HFV_ADJ_FLUX_ERROR);
 }
 

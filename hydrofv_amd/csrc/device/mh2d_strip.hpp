@@ -659,7 +659,7 @@ __device__ __forceinline__ bool uniform_strip(const Grid2D& G, double dt, const 
 // integrator_godunov_first_order.c:362-497, whose faces take the cells' own
 // primitive states; its sweeps, update order and source term are those of
 // the MUSCL-Hancock step, and it needs one ghost layer, not two.
-template <int LIM, int SOLVER, bool WMODE, bool BRANCHY, bool DEFER = false>
+template <int LIM, int SOLVER, bool WMODE, bool BRANCHY, bool DEFER = false, bool UNIFORM = true>
 __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, const Fields4& U, const Fields4& V,
                                                   const Fields4& W, int sx, int sy, int lane,
                                                   const Deferred& D = Deferred(), int sub = 0, int nsub = 1)
@@ -698,8 +698,9 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     constexpr int AHEAD = (LIM == FIRST_ORDER) ? 1 : 2;  // the row a step's iteration r reads
     // A tile of one state takes the results of one cell (tile_uniform,
     // uniform_strip); the first step of a run, which reads the primitive
-    // state, does not take this form.
-    if constexpr (!WMODE) {
+    // state, does not take this form, nor does a kernel built without it
+    // (UNIFORM false: the same results, fewer registers).
+    if constexpr (!WMODE && UNIFORM) {
         RawCell uref;
         if (!G.full_tiles && tile_uniform<AHEAD>(G, U, y0, y1, col, uref) &&
             uniform_strip<LIM, SOLVER, BRANCHY, DEFER>(G, dt, U, V, uref, y0, y1, col, out_col, ghost_lo_col,
