@@ -58,4 +58,28 @@ inline int balanced_strip_rows(int nx, int ny, long long slots)
     return best;
 }
 
+// The exact solver's default (its split form, mh2d.hip): most of its strip
+// pass is tiles of one state, whose cost is their loads and stores, so fewer,
+// taller strips (fewer halo rows re-read, fewer rounds) pay where a grid has
+// at least half a round of 32-row strips: the height in [28, 40] that fills
+// its rounds best (ties: nearest 32), e.g. 36 rows for 2048^2 on 2048 slots
+// (1995 strips, 97% of one round; the Sedov run 2.68 -> 2.60 s, where 18 rows
+// are the general default).  Smaller grids take balanced_strip_rows.
+inline int exact_strip_rows(int nx, int ny, long long slots)
+{
+    const long long nsx = (nx + STRIP_COLS_2D - 1) / STRIP_COLS_2D;
+    if (slots <= 0 || 2 * nsx * ((ny + 31) / 32) < slots) return balanced_strip_rows(nx, ny, slots);
+    int best = 32;
+    double best_fill = strip_round_fill(nsx * ((ny + 31) / 32), slots);
+    for (int r = 28; r <= 40; r++) {
+        const double f = strip_round_fill(nsx * ((ny + r - 1) / r), slots);
+        const int dr = r > 32 ? r - 32 : 32 - r, db = best > 32 ? best - 32 : 32 - best;
+        if (f > best_fill + 1e-3 || (f >= best_fill - 1e-3 && dr < db)) {
+            best = r;
+            best_fill = f;
+        }
+    }
+    return best;
+}
+
 }  // namespace hfv

@@ -1039,7 +1039,9 @@ void grid_from_config(const hfv_mh2d_config* cfg, long long pitch, Grid2D& G, Ru
     G.volume = cfg->volume;
     for (int s = 0; s < 4; s++) G.bc[s] = cfg->bc[s];
     G.nsx = (G.nx + STRIP_COLS - 1) / STRIP_COLS;
-    G.strip_rows = cfg->strip_rows > 0 ? cfg->strip_rows : balanced_strip_rows(G.nx, G.ny, resident_step_waves(cfg));
+    G.strip_rows = cfg->strip_rows > 0                ? cfg->strip_rows
+                   : cfg->riemann_solver == EXACT ? exact_strip_rows(G.nx, G.ny, resident_step_waves(cfg))
+                                                  : balanced_strip_rows(G.nx, G.ny, resident_step_waves(cfg));
     G.nsy = (G.ny + G.strip_rows - 1) / G.strip_rows;
     G.full_tiles = cfg->full_tiles != 0;
     rp = RunParams();

@@ -228,5 +228,6 @@ void hc_c2p_vjp(double gamma, double vol, long long n, const double* U, const do
 // the 2D step's strip height for an nx x ny grid on a device holding `slots`
 // step wavefronts at once
 int hc_strip_rows(int nx, int ny, long long slots) { return hfv::balanced_strip_rows(nx, ny, slots); }
+int hc_exact_strip_rows(int nx, int ny, long long slots) { return hfv::exact_strip_rows(nx, ny, slots); }
 
 }  // extern "C"

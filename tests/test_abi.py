@@ -208,6 +208,16 @@ def test_strip_height_fills_the_rounds_of_resident_wavefronts():
         window = range(base - base // 4, base + base // 4 + 1)
         assert r in window
         assert fill(strips(n, r), 2048) >= max(fill(strips(n, q), 2048) for q in window) - 1e-3
+    # the exact solver's default: [28, 40] rows once a grid has half a round of 32-row strips
+    L.hc_exact_strip_rows.argtypes = [C.c_int, C.c_int, C.c_longlong]
+    L.hc_exact_strip_rows.restype = C.c_int
+    assert L.hc_exact_strip_rows(2048, 2048, 2048) == 36    # 1995 strips: 97% of one round
+    assert L.hc_exact_strip_rows(256, 256, 2048) == L.hc_strip_rows(256, 256, 2048)  # small: the general rule
+    assert L.hc_exact_strip_rows(2048, 2048, 0) == 16       # device unknown
+    for n in (1500, 2048, 4096):
+        r = L.hc_exact_strip_rows(n, n, 2048)
+        assert 28 <= r <= 40
+        assert fill(strips(n, r), 2048) >= max(fill(strips(n, q), 2048) for q in range(28, 41)) - 1e-3
 
 
 def test_device_bc_registry():
