@@ -50,7 +50,7 @@ struct Grid2D {
     int bc[4];           // x_min, x_max, y_min, y_max
     int strip_rows;
     int nsx, nsy;        // strips in x and y
-    int full_tiles = 0;  // 1: no uniform-tile form (tile_uniform)
+    int full_tiles = 0;  // 1: no uniform-row form (uniform_rows)
 };
 
 // Boundary mapping of a ghost column (row) index onto the interior cell whose
@@ -506,71 +506,50 @@ struct StripResult {
     long long err_cell;
 };
 
-// True when every cell that the rows [y0, y1) of a strip read -- rows
-// y0 - AHEAD .. y1 + AHEAD - 1 of the wavefront's 64 columns, as load_raw
-// maps them -- holds the same conserved state as lane 0's first one, bit for
-// bit, with the same velocity flips; ref: the lane's first cell as loaded
-// (that state, when true).  Then every cell and face of those rows performs
-// the same operations on the same values (uniform_strip).  The first row is
-// tested alone, then rows 16 at a time (the last row repeated past the end),
-// and the test stops at the first batch that holds a difference: on a tile
-// without a uniform region it costs one row's loads, which the step reads
-// again from L2.
-template <int AHEAD>
-__device__ __forceinline__ bool tile_uniform(const Grid2D& G, const Fields4& U, int y0, int y1, int col,
-                                             RawCell& ref)
+// Rows of one state (the quiet part of a blast, a quadrant's interior): a
+// row's results depend only on the cells of its stencil, rows r - AHEAD ..
+// r + AHEAD of the wavefront's 64 columns as load_raw maps them.  When all
+// of those hold, bit for bit, the same conserved state (and velocity flips)
+// as lane 0's first cell, every cell and face of the row performs the
+// operations of one cell on the same values.  uniform_rows computes those
+// results once, from that state, by the calls the row loop of strip_step
+// makes, in its order, with every neighbour equal to the cell itself (a
+// neighbour exchange then returns the lane's own value, and every slope is
+// limit(0, 0)); then it verifies the strip's rows 16 at a time and stores
+// each row whose stencil is verified as soon as it is: the interior cells'
+// new state, the ghost columns' one-face updates (each from its own stored
+// state) and, at the grid's bottom and top, the ghost rows' one-face
+// updates.  Returns the first row it has not stored (y0: none, when lane
+// 0's first cell is not every lane's, or when a face would be queued
+// (DEFER) or fails); the caller runs the rows from there as usual.  *smx,
+// *smy: the stored cells' CFL maxima.  Every lane must call this
+// (wavefront-wide guesses and powers in HLLC).
+template <int LIM, int SOLVER, bool BRANCHY, bool DEFER>
+__device__ __forceinline__ int uniform_rows(const Grid2D& G, double dt, const Fields4& U, const Fields4& V,
+                                            int y0, int y1, int col, bool out_col, bool ghost_lo_col,
+                                            bool ghost_hi_col, double cx, double cy, double rvol, bool vol_ok,
+                                            double* trrs_q, int lane, const double* pow_lds, double* smx,
+                                            double* smy)
 {
+    constexpr int AHEAD = (LIM == FIRST_ORDER) ? 1 : 2;
     const Fields4 none{};
-    load_raw<false>(G, U, none, y0 - AHEAD, col, ref);
+    RawCell rc;
+    load_raw<false>(G, U, none, y0 - AHEAD, col, rc);
     unsigned long long rb[4];
     bool same = true;
 #pragma unroll
     for (int f = 0; f < 4; f++) {
-        const unsigned long long b = (unsigned long long)__double_as_longlong(ref.v[f]);
+        const unsigned long long b = (unsigned long long)__double_as_longlong(rc.v[f]);
         const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
         const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
         rb[f] = ((unsigned long long)hi << 32) | lo;
         same &= b == rb[f];
     }
-    const int fu0 = __builtin_amdgcn_readfirstlane(ref.fu), fv0 = __builtin_amdgcn_readfirstlane(ref.fv);
-    same &= (ref.fu == fu0) & (ref.fv == fv0);
-    if (!wave_all(same)) return false;
-    constexpr int BATCH = 16;
-    const int last = y1 + AHEAD - 1;
-    for (int q = y0 - AHEAD + 1; q <= last; q += BATCH) {
-        RawCell c[BATCH];
-#pragma unroll
-        for (int j = 0; j < BATCH; j++) load_raw<false>(G, U, none, min(q + j, last), col, c[j]);
-#pragma unroll
-        for (int j = 0; j < BATCH; j++) {
-#pragma unroll
-            for (int f = 0; f < 4; f++) same &= (unsigned long long)__double_as_longlong(c[j].v[f]) == rb[f];
-            same &= (c[j].fu == fu0) & (c[j].fv == fv0);
-        }
-        if (!wave_all(same)) return false;
-    }
-    return true;
-}
+    const int fu0 = __builtin_amdgcn_readfirstlane(rc.fu), fv0 = __builtin_amdgcn_readfirstlane(rc.fv);
+    same &= (rc.fu == fu0) & (rc.fv == fv0);
+    if (!wave_all(same)) return y0;
 
-// A uniform tile (tile_uniform): every cell and face of rows [y0, y1)
-// performs the operations of one cell on the same values, so the results
-// are computed once, from the tile's state rc, by the calls the row loop of
-// strip_step makes, in its order, with every neighbour equal to the cell
-// itself (a neighbour exchange then returns the lane's own value, and every
-// slope is limit(0, 0)), and stored in every row: the interior cells' new
-// state, the ghost columns' one-face updates (each from its own stored
-// state) and the ghost rows' one-face updates at the grid's bottom and top.
-// Returns false, having written nothing, when a face would be queued (DEFER)
-// or fails: the caller then runs the rows as usual.  *smx, *smy: the updated
-// cells' CFL maxima.  Every lane must call this (wavefront-wide guesses and
-// powers in HLLC).
-template <int LIM, int SOLVER, bool BRANCHY, bool DEFER>
-__device__ __forceinline__ bool uniform_strip(const Grid2D& G, double dt, const Fields4& U, const Fields4& V,
-                                              const RawCell& rc, int y0, int y1, int col, bool out_col,
-                                              bool ghost_lo_col, bool ghost_hi_col, double cx, double cy,
-                                              double rvol, bool vol_ok, double* trrs_q, int lane,
-                                              const double* pow_lds, double* smx, double* smy)
-{
+    // the results of a cell whose stencil holds rc's state
     const double gamma = G.gamma, tol = G.tol;
     double w[4];
     raw_to_prim<false>(G, rc, w, rvol, vol_ok);
@@ -596,13 +575,13 @@ __device__ __forceinline__ bool uniform_strip(const Grid2D& G, double dt, const 
     } else if constexpr (DEFER) {
         const bool qy = direct_face<true>(gamma, tol, yhi, ylo, fy);
         const bool qx = direct_face<false>(gamma, tol, xhi, xlo, fx);
-        if (!wave_all(!qy && !qx)) return false;
+        if (!wave_all(!qy && !qx)) return y0;
         ey = ex = ERR_NONE;
     } else {
         ey = yflux<SOLVER>(gamma, tol, yhi, ylo, fy);
         ex = flux<SOLVER, 4>(gamma, tol, xhi, xlo, fx);
     }
-    if (!wave_all((ey == ERR_NONE) & (ex == ERR_NONE))) return false;
+    if (!wave_all((ey == ERR_NONE) & (ex == ERR_NONE))) return y0;
     // an interior cell's update (integrator_muscl_hancock.c:806-814,
     // 1092-1100; source_term.c:213-240) and its CFL maxima (utils.c:75-92)
     double u[4];
@@ -616,39 +595,69 @@ __device__ __forceinline__ bool uniform_strip(const Grid2D& G, double dt, const 
         u[f] = x;
     }
     if (G.gravity != 0.0) gravity_source(dt, G.gravity, u[0], &u[2], &u[3]);
+    double sx_ = -INFINITY, sy_ = -INFINITY;
     if (out_col) {
         double wn[4];
         c2p_2d_rn(gamma, G.volume, rvol, vol_ok, u[0], u[1], u[2], u[3], wn);
         const double a = sound_speed_rn(gamma, wn[0], wn[3]);
-        const double sx_ = a + fabs(wn[1]);
-        const double sy_ = a + fabs(wn[2]);
+        sx_ = a + fabs(wn[1]);
+        sy_ = a + fabs(wn[2]);
+    }
+
+    // verify the rows 16 at a time (the last row repeated past the end) and
+    // store the rows whose stencils are verified
+    constexpr int BATCH = 16;
+    const int last = y1 + AHEAD - 1;
+    int verified = y0 - AHEAD;  // rows y0 - AHEAD .. verified hold the state
+    int stored = y0;            // rows y0 .. stored - 1 are stored
+    for (int q = y0 - AHEAD + 1; q <= last && verified == q - 1; q += BATCH) {
+        RawCell c[BATCH];
+#pragma unroll
+        for (int j = 0; j < BATCH; j++) load_raw<false>(G, U, none, min(q + j, last), col, c[j]);
+        int nok = 0;  // the batch's leading rows that hold the state
+        bool run = true;
+#pragma unroll
+        for (int j = 0; j < BATCH; j++) {
+            bool m = (c[j].fu == fu0) & (c[j].fv == fv0);
+#pragma unroll
+            for (int f = 0; f < 4; f++) m &= (unsigned long long)__double_as_longlong(c[j].v[f]) == rb[f];
+            run = run && wave_all(m);
+            nok += run ? 1 : 0;
+        }
+        verified = min(q + nok - 1, last);
+        const int upto = min(y1, verified - AHEAD + 1);
+        if (upto > stored && stored == G.g && out_col) {
+            // the bottom ghost row's update from the y_min face (1092-1095)
+            const long long k = (long long)(G.g - 1) * G.pitch + col;
+#pragma unroll
+            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] - fy[f] * dt * G.area_y;
+        }
+        for (int r = stored; r < upto; r++) {
+            const long long k = (long long)r * G.pitch + col;
+            if (out_col) {
+#pragma unroll
+                for (int f = 0; f < 4; f++) V.f[f][k] = u[f];
+            } else if (ghost_lo_col) {
+#pragma unroll
+                for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] - fx[f] * dt * G.area_x;
+            } else if (ghost_hi_col) {
+#pragma unroll
+                for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] + fx[f] * dt * G.area_x;
+            }
+        }
+        if (upto == y1 && stored < y1 && y1 == G.g + G.ny && out_col) {
+            // the top ghost row's update from the y_max face (1097-1100)
+            const long long k = (long long)(G.g + G.ny) * G.pitch + col;
+#pragma unroll
+            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] + fy[f] * dt * G.area_y;
+        }
+        if (upto > stored) stored = upto;
+    }
+    if (stored > y0) {
         if (sx_ > *smx) *smx = sx_;
         if (sy_ > *smy) *smy = sy_;
     }
-    for (int q = y0; q < y1; q++) {
-        const long long k = (long long)q * G.pitch + col;
-        if (out_col) {
-#pragma unroll
-            for (int f = 0; f < 4; f++) V.f[f][k] = u[f];
-        } else if (ghost_lo_col) {
-#pragma unroll
-            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] - fx[f] * dt * G.area_x;
-        } else if (ghost_hi_col) {
-#pragma unroll
-            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] + fx[f] * dt * G.area_x;
-        }
-    }
-    if (y0 == G.g && out_col) {
-        const long long k = (long long)(G.g - 1) * G.pitch + col;
-#pragma unroll
-        for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] - fy[f] * dt * G.area_y;
-    }
-    if (y1 == G.g + G.ny && out_col) {
-        const long long k = (long long)(G.g + G.ny) * G.pitch + col;
-#pragma unroll
-        for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] + fy[f] * dt * G.area_y;
-    }
-    return true;
+    return stored;
 }
 
 // One strip (sx, sy) of the step U -> V with time step dt, or (nsub > 1) part
@@ -696,22 +705,24 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     __shared__ double pow_lds[POW_TABS_LEN];
     if constexpr (SOLVER == HLLC) stage_pow_tables(pow_lds, lane);
     constexpr int AHEAD = (LIM == FIRST_ORDER) ? 1 : 2;  // the row a step's iteration r reads
-    // A tile of one state takes the results of one cell (tile_uniform,
-    // uniform_strip); the first step of a run, which reads the primitive
-    // state, does not take this form, nor does a kernel built without it
-    // (UNIFORM false: the same results, fewer registers).
+    // Leading rows of one state take the results of one cell (uniform_rows),
+    // and the rows from the first it has not stored run as usual; the first
+    // step of a run, which reads the primitive state, does not take this
+    // form, nor does a kernel built without it (UNIFORM false: the same
+    // results, fewer registers).
     if constexpr (!WMODE && UNIFORM) {
-        RawCell uref;
-        if (!G.full_tiles && tile_uniform<AHEAD>(G, U, y0, y1, col, uref) &&
-            uniform_strip<LIM, SOLVER, BRANCHY, DEFER>(G, dt, U, V, uref, y0, y1, col, out_col, ghost_lo_col,
-                                                        ghost_hi_col, cx, cy, rvol, vol_ok, trrs_q, lane, pow_lds,
-                                                        &smx, &smy)) {
-            StripResult res;
-            res.smx = smx;
-            res.smy = smy;
-            res.err = ERR_NONE;
-            res.err_cell = -1;
-            return res;
+        if (!G.full_tiles) {
+            y0 = uniform_rows<LIM, SOLVER, BRANCHY, DEFER>(G, dt, U, V, y0, y1, col, out_col, ghost_lo_col,
+                                                           ghost_hi_col, cx, cy, rvol, vol_ok, trrs_q, lane, pow_lds,
+                                                           &smx, &smy);
+            if (y0 >= y1) {
+                StripResult res;
+                res.smx = smx;
+                res.smy = smy;
+                res.err = ERR_NONE;
+                res.err_cell = -1;
+                return res;
+            }
         }
     }
     // The conserved states of the last AHEAD + 1 rows loaded, each lane its
