@@ -365,25 +365,33 @@ __device__ __forceinline__ void hllc_row_faces(double gamma, double tol, const d
 }
 
 // Neighbour exchange across the wavefront by the GFX9 whole-wavefront DPP
-// shifts (v_mov_b32_dpp wave_shr:1 / wave_shl:1), register to register, two
-// 32-bit moves per double: lane i receives lane i-1's (from_left) or lane
-// i+1's (from_right) value; the edge lane with no neighbour keeps its own,
-// as with __shfl_up / __shfl_down over the 64 lanes.
-constexpr int DPP_WAVE_SHL1 = 0x130;
-constexpr int DPP_WAVE_SHR1 = 0x138;
+// rotations (v_mov_b32_dpp wave_ror:1 / wave_rol:1), register to register,
+// one 32-bit move per half of a double: lane i receives lane i-1's
+// (from_left) or lane i+1's (from_right) value, and the edge lane the value
+// of the lane at the other edge.  A rotation writes every lane, so the
+// destination needs no copy of the old value first (a shift would, for the
+// edge lane it leaves alone).  The edge lanes are outer halo columns of a
+// strip (lane 0: column x0 - 2, lane 63: x0 + 61); what they receive feeds
+// only their own slopes, the faces left and right of x0 - 2 and left of
+// x0 + 61, and the right-face fluxes of lanes 62 and 63.  No cell that a
+// strip updates and no face that it reports, queues or stores uses these
+// (strip_step's lane roles); a wavefront-wide range test that one of them
+// fails sends its group down the library form, whose bits are the same.
+constexpr int DPP_WAVE_ROL1 = 0x134;
+constexpr int DPP_WAVE_ROR1 = 0x13C;
 
 template <int CTRL>
-__device__ __forceinline__ double dpp_shift(double v)
+__device__ __forceinline__ double dpp_rotate(double v)
 {
     const long long b = __double_as_longlong(v);
     const int lo = (int)b, hi = (int)(b >> 32);
-    const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
-    const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    const int lo2 = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false);
+    const int hi2 = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false);
     return __longlong_as_double(((long long)hi2 << 32) | (unsigned int)lo2);
 }
 
-__device__ __forceinline__ double shfl_from_left(double v) { return dpp_shift<DPP_WAVE_SHR1>(v); }
-__device__ __forceinline__ double shfl_from_right(double v) { return dpp_shift<DPP_WAVE_SHL1>(v); }
+__device__ __forceinline__ double shfl_from_left(double v) { return dpp_rotate<DPP_WAVE_ROR1>(v); }
+__device__ __forceinline__ double shfl_from_right(double v) { return dpp_rotate<DPP_WAVE_ROL1>(v); }
 
 // DEFER (exact solver): a face whose two states differ (its Riemann problem
 // needs the pressure iteration, or the vacuum sampler) is not solved by the
