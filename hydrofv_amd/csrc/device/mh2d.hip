@@ -334,12 +334,92 @@ __device__ PStar newton_finish_wave(const PressureFn& F, double tol, const Newto
     }
 }
 
-// The bracket scan and bisection of each set-aside face, one face per
-// wavefront (Deferred::spp, sprc): newton_finish of the state the face was
-// set aside with, which k_solve_join takes for the faces whose Newton loop
-// k_solve_slow ends without converging and with the bracket unchanged.  The
-// queue length and the states were written by k_solve_faces; this launch
-// may run beside k_solve_slow's.
+// One side's wave function f_K (df false) or its derivative f'_K (df true)
+// at ps, by the expression pressure_fn_df evaluates for it (shock_f_df,
+// rarefaction_f_df; exact.c:512-565), computed alone: y is the value's
+// rarefaction exponent (F.zf for f_K, F.zd for f'_K), and pow_pos takes the
+// logarithm that rarefaction_f_df shares between its two powers.
+__device__ __forceinline__ double wave_fn_part(const PressureFn& F, const WaveSide& S, double y, bool df, double ps)
+{
+    if (ps > S.p) {
+        const double s = sqrt(S.A / (ps + S.B));
+        const double d = ps - S.p;
+        return df ? (1.0 - 0.5 * d / (S.B - ps)) * s : d * s;
+    }
+    const double r = pow_pos(ps / S.p, y, F.lt, F.et);
+    // r / p a, or 2 a (r - 1) / (gamma - 1): one division either way
+    const double num = df ? r : S.a2 * (r - 1.0);
+    return num / (df ? S.pa : F.gm1);
+}
+
+// v of lane K of the calling lane's quad (lanes 4q .. 4q + 3), by a DPP
+// quad_perm broadcast; the four lanes of the quad must be active.
+template <int K>
+__device__ __forceinline__ double quad_lane(double v)
+{
+    constexpr int CTRL = K * 0x55;  // quad_perm [K, K, K, K]
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// newton_resume (hfv_math.hpp) by a quad of lanes: each iteration's f_0,
+// f'_0, f_1 and f'_1 (the sides in PressureFn's order) are computed by lanes
+// 0, 1, 2 and 3 of the quad (lane & 3) and summed in pressure_fn_df's order,
+// so every lane of the quad takes newton_run's iterates, and the latency of
+// an iteration is one side's value instead of all four in turn.  Returns 2
+// with *P when the loop converged, 1 when it bracketed a root it had not
+// bracketed when it started (*P: newton_finish's result), 0 when
+// newton_finish of the state it started from applies.  The four lanes of a
+// quad call this with the same face.
+__device__ int newton_resume_quad(const PressureFn& F, double tol, NewtonState st, PStar* P, int lane)
+{
+    const int role = lane & 3;
+    const bool df = (role & 1) != 0;
+    const WaveSide S = (role & 2) ? F.s1 : F.s0;
+    const double y = df ? F.zd : F.zf;
+    const int br0 = st.bracketed;
+    for (; st.it < 100; st.it++) {
+        double f, d;
+        if (st.it == 0) {
+            f = st.f_hi;
+            d = st.df0;
+        } else {
+            const double v = wave_fn_part(F, S, y, df, st.p0);
+            const double v0 = quad_lane<0>(v), v1 = quad_lane<1>(v);
+            const double v2 = quad_lane<2>(v), v3 = quad_lane<3>(v);
+            f = v0 + v2 + F.du;
+            d = v1 + v3;
+        }
+        if (!st.bracketed && f * st.f_hi < 0.0) {
+            st.bracketed = 1;
+            st.f_lo = f;
+            st.p_lo = st.p0;
+        }
+        const double p1 = st.p0 - f / d;
+        if (p1 < 0.0) break;
+        if (2.0 * fabs(p1 - st.p0) / (p1 + st.p0) < tol) {
+            *P = PStar{p1, ERR_NONE};
+            return 2;
+        }
+        st.p0 = p1;
+    }
+    if (st.bracketed == br0) return 0;
+    *P = newton_finish(F, tol, st);
+    return 1;
+}
+
+// k_solve_slow's outcome of a face whose result is newton_finish of its
+// set-aside state (k_solve_spec's)
+constexpr int SLOW_TAKE_SPEC = -100;
+
+// The bracket scan and bisection of the set-aside faces whose Newton loop
+// k_solve_slow ended without converging and with the bracket unchanged (sit
+// SLOW_TAKE_SPEC), one face per wavefront: newton_finish of the state the
+// face was set aside with, to Deferred::spp and sprc.  The guess, f there and
+// the bracket (sst[1..4], sbr) are k_solve_faces's; k_solve_slow, which ran
+// before this launch, wrote only sst[0] and sit.
 __global__ void __launch_bounds__(LANES) k_solve_spec(Grid2D G, Deferred D0, int slot)
 {
     const Deferred D = defer_slot(D0, slot);
@@ -352,6 +432,7 @@ __global__ void __launch_bounds__(LANES) k_solve_spec(Grid2D G, Deferred D0, int
     __shared__ double pow_lds[POW_TABS_LEN];
     stage_pow_tables(pow_lds, lane);
     for (long long j = blockIdx.x; j < n; j += gridDim.x) {
+        if (D.sit[j] != SLOW_TAKE_SPEC) continue;
         const long long i = D.sidx[j];
         const double rl = D.qw[0][i], ul = D.qw[1][i], pl = D.qw[3][i];
         const double rr = D.qw[4][i], ur = D.qw[5][i], pr = D.qw[7][i];
@@ -359,9 +440,8 @@ __global__ void __launch_bounds__(LANES) k_solve_spec(Grid2D G, Deferred D0, int
         const double ar = (rr > 0.0) ? sound_speed(G.gamma, rr, pr) : 0.0;
         const Side L{rl, ul, pl, al}, R{rr, ur, pr, ar};
         const PressureFn F = pressure_fn_setup(G.gamma, L, R, pow_lds, pow_lds + POW_LOG_TAB_LEN);
-        // newton_finish reads the guess, f there and the bracket; the
-        // iterate and the iteration count (sst[0], sit) are k_solve_slow's,
-        // which may run at the same time and overwrite them
+        // newton_finish reads the guess, f there and the bracket, as the
+        // face was set aside with them
         NewtonState st;
         st.p0 = 0.0;
         st.p_hi = D.sst[1][j];
@@ -379,25 +459,27 @@ __global__ void __launch_bounds__(LANES) k_solve_spec(Grid2D G, Deferred D0, int
     }
 }
 
-// k_solve_slow's outcome of a face whose result is newton_finish of its
-// set-aside state (k_solve_spec's)
-constexpr int SLOW_TAKE_SPEC = -100;
+// Set-aside faces per wavefront of k_solve_slow: one per quad of lanes
+constexpr int SLOW_FACES_PER_WAVE = LANES / 4;
 
-// The set-aside faces' Newton loops, resumed from their saved state (one
-// face per lane): the outcome goes to sst[0] (p*) and sit (the failure kind,
-// or SLOW_TAKE_SPEC).  A face that brackets a root in this loop is finished
-// here, serially.
+// The set-aside faces' Newton loops, resumed from their saved state, one
+// face per quad of lanes (newton_resume_quad): the outcome goes to sst[0]
+// (p*) and sit (the failure kind, or SLOW_TAKE_SPEC).  A face that brackets
+// a root in this loop is finished here, serially.  A set-aside face has
+// taken at least one iteration (k_solve_faces' cap is at least 1).
 __global__ void __launch_bounds__(LANES) k_solve_slow(Grid2D G, Deferred D0, int slot)
 {
     const Deferred D = defer_slot(D0, slot);
     const long long n = (long long)min(D.count[2], (unsigned long long)D.slow_cap);
-    if ((long long)blockIdx.x * LANES >= n) return;
+    if ((long long)blockIdx.x * SLOW_FACES_PER_WAVE >= n) return;
     // these waves are a latency-bound chain that runs beside the strip pass:
     // their instructions win the SIMD's issue arbitration when ready
     __builtin_amdgcn_s_setprio(3);
+    const int lane = threadIdx.x;
     __shared__ double pow_lds[POW_TABS_LEN];
-    stage_pow_tables(pow_lds, threadIdx.x);
-    for (long long j = (long long)blockIdx.x * LANES + threadIdx.x; j < n; j += (long long)gridDim.x * LANES) {
+    stage_pow_tables(pow_lds, lane);
+    const long long stride = (long long)gridDim.x * SLOW_FACES_PER_WAVE;
+    for (long long j = (long long)blockIdx.x * SLOW_FACES_PER_WAVE + lane / 4; j < n; j += stride) {
         const long long i = D.sidx[j];
         const double rl = D.qw[0][i], ul = D.qw[1][i], pl = D.qw[3][i];
         const double rr = D.qw[4][i], ur = D.qw[5][i], pr = D.qw[7][i];
@@ -415,11 +497,14 @@ __global__ void __launch_bounds__(LANES) k_solve_slow(Grid2D G, Deferred D0, int
         st.it = D.sit[j];
         st.bracketed = D.sbr[j];
         PStar P;
-        if (newton_resume(F, G.tol, st, &P)) {
-            D.sst[0][j] = P.p;
-            D.sit[j] = P.rc;
-        } else {
-            D.sit[j] = SLOW_TAKE_SPEC;
+        const int took = newton_resume_quad(F, G.tol, st, &P, lane);
+        if ((lane & 3) == 0) {
+            if (took) {
+                D.sst[0][j] = P.p;
+                D.sit[j] = P.rc;
+            } else {
+                D.sit[j] = SLOW_TAKE_SPEC;
+            }
         }
     }
 }
@@ -731,6 +816,48 @@ __global__ void __launch_bounds__(LANES) k_probe_finish(double gamma, double tol
     }
 }
 
+// hfv_probe_exact_finish impl 2: Newton for cap iterations by one lane,
+// the rest of the loop as k_solve_slow takes it (newton_resume_quad, a quad
+// of lanes per face), then newton_finish where its outcome says so.
+__global__ void __launch_bounds__(LANES) k_probe_resume_quad(double gamma, double tol, int cap, long long n,
+                                                             const double* wl, const double* wr, double* p, int* rc)
+{
+    const int lane = threadIdx.x;
+    const long long i = (long long)blockIdx.x * SLOW_FACES_PER_WAVE + lane / 4;
+    if (i >= n) return;
+    const double rl = wl[4 * i], ul = wl[4 * i + 1], pl = wl[4 * i + 3];
+    const double rr = wr[4 * i], ur = wr[4 * i + 1], pr = wr[4 * i + 3];
+    const double al = (rl > 0.0) ? sound_speed(gamma, rl, pl) : 0.0;
+    const double ar = (rr > 0.0) ? sound_speed(gamma, rr, pr) : 0.0;
+    double out_p = 0.0;
+    int out_rc = -3;
+    if (!(rl <= 0.0 || rr <= 0.0 || ((al + ar) * 2.0 / (gamma + 1.0)) <= (ur - ul))) {
+        const Side L{rl, ul, pl, al}, R{rr, ur, pr, ar};
+        const PressureFn F = pressure_fn_setup(gamma, L, R);
+        NewtonState st = newton_begin(gamma, L, R, tol, F);
+        double ps;
+        const int r = newton_run(F, tol, st, cap, &ps);
+        PStar P{0.0, ERR_NONE};
+        if (r > 0) {
+            out_p = ps;
+            out_rc = -1;
+        } else if (r < 0) {
+            P = newton_finish(F, tol, st);
+            out_p = P.p;
+            out_rc = P.rc;
+        } else {
+            const int took = newton_resume_quad(F, tol, st, &P, lane);
+            if (took == 0) P = newton_finish(F, tol, st);
+            out_p = P.p;
+            out_rc = (took == 2) ? -1 : P.rc;
+        }
+    }
+    if ((lane & 3) == 0) {
+        p[i] = out_p;
+        rc[i] = out_rc;
+    }
+}
+
 // Working memory of hfv_mh2d_run_noalloc on each GPU: the three controller
 // slots in its first 256 bytes, then the second conserved buffer.  Host code
 // takes its address per device with hipGetSymbolAddress (scratch_base) and
@@ -771,8 +898,7 @@ struct hfv_mh2d {
     unsigned long long* count_all;  // exact solver: both views' queue counters, [view][slot][3]
     int* plan;             // exact solver: the split of the tiles (plan_tiles)
     hipStream_t aux;       // exact solver: the second stream of the split form
-    hipStream_t aux2;      // exact solver: its third stream
-    hipEvent_t ev_fork, ev_join, ev_join2;
+    hipEvent_t ev_fork, ev_join;
     long long defer_clear; // exact solver: the launch whose queue counters are known to be clear
     double* defer_slab;
 };
@@ -841,7 +967,9 @@ void launch_exact(hfv_mh2d* h, bool wmode, hipStream_t s)
     const long long cells = (long long)G.tx * G.ty;
     const unsigned solve_blocks = (unsigned)std::min<long long>(4096, (2 * cells + LANES - 1) / LANES);
     const unsigned update_blocks = (unsigned)std::min<long long>(1024, (cells + UPD_THREADS - 1) / UPD_THREADS);
-    const unsigned slow_blocks = (unsigned)std::min<long long>(1024, (h->D.slow_cap + LANES - 1) / LANES);
+    const unsigned slow_blocks =
+        (unsigned)std::min<long long>(4096, (h->D.slow_cap + SLOW_FACES_PER_WAVE - 1) / SLOW_FACES_PER_WAVE);
+    const unsigned join_blocks = (unsigned)std::min<long long>(1024, (h->D.slow_cap + LANES - 1) / LANES);
     const unsigned spec_blocks = (unsigned)std::min<long long>(2048, h->D.slow_cap);
     const int cap_a = std::min(ntiles, DEFER_MAX_A);
     if (h->defer_clear != h->launches)
@@ -856,29 +984,26 @@ void launch_exact(hfv_mh2d* h, bool wmode, hipStream_t s)
             hipLaunchKernelGGL((k_mh2d_step_defer<LIM, false>), dim3(ntiles), block, 0, s, G, h->rp, h->U[0],
                                h->U[1], h->W, h->ctrl, slot, h->D, h->count_all);
         hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), block, 0, s, G, h->D, h->ctrl, slot, NEWTON_CAP);
-        hipLaunchKernelGGL(k_solve_spec, dim3(spec_blocks), block, 0, s, G, h->D, slot);
         hipLaunchKernelGGL(k_solve_slow, dim3(slow_blocks), block, 0, s, G, h->D, slot);
-        hipLaunchKernelGGL(k_solve_join, dim3(slow_blocks), block, 0, s, G, h->D, h->ctrl, slot);
+        hipLaunchKernelGGL(k_solve_spec, dim3(spec_blocks), block, 0, s, G, h->D, slot);
+        hipLaunchKernelGGL(k_solve_join, dim3(join_blocks), block, 0, s, G, h->D, h->ctrl, slot);
         const int planner = ntiles <= DEFER_PLAN_MAX_TILES;
         hipLaunchKernelGGL(k_update_deferred, dim3(update_blocks + planner), ublock, 0, s, G, h->rp, h->U[0],
                            h->U[1], h->D, h->DB, 1, h->ctrl, slot, h->plan, cap_a, planner);
     } else {
         // split form: pass A and its first face pass on s; its set-aside
-        // faces' Newton loops on aux and their speculative finish on aux2,
-        // joined on aux, while pass B runs on s and solves its own faces
-        // (rarely any: a tile with none in the previous step and none next to
-        // it) in their first pass
+        // faces' Newton loops, the finish of those that need it and the join
+        // on aux, while pass B runs on s and solves its own faces (rarely
+        // any: a tile with none in the previous step and none next to it) in
+        // their first pass
         hipLaunchKernelGGL((k_mh2d_step_split<LIM, 0>), dim3(cap_a * DEFER_NSUB), block, 0, s, G, h->rp, h->U[0],
                            h->U[1], h->ctrl, slot, h->D, h->count_all, (const int*)h->plan);
         hipLaunchKernelGGL(k_solve_faces, dim3(solve_blocks), block, 0, s, G, h->D, h->ctrl, slot, NEWTON_CAP);
         (void)hipEventRecord(h->ev_fork, s);
         (void)hipStreamWaitEvent(h->aux, h->ev_fork, 0);
-        (void)hipStreamWaitEvent(h->aux2, h->ev_fork, 0);
         hipLaunchKernelGGL(k_solve_slow, dim3(slow_blocks), block, 0, h->aux, G, h->D, slot);
-        hipLaunchKernelGGL(k_solve_spec, dim3(spec_blocks), block, 0, h->aux2, G, h->D, slot);
-        (void)hipEventRecord(h->ev_join2, h->aux2);
-        (void)hipStreamWaitEvent(h->aux, h->ev_join2, 0);
-        hipLaunchKernelGGL(k_solve_join, dim3(slow_blocks), block, 0, h->aux, G, h->D, h->ctrl, slot);
+        hipLaunchKernelGGL(k_solve_spec, dim3(spec_blocks), block, 0, h->aux, G, h->D, slot);
+        hipLaunchKernelGGL(k_solve_join, dim3(join_blocks), block, 0, h->aux, G, h->D, h->ctrl, slot);
         (void)hipEventRecord(h->ev_join, h->aux);
         hipLaunchKernelGGL((k_mh2d_step_split<LIM, 1>), dim3(ntiles), block, 0, s, G, h->rp, h->U[0], h->U[1],
                            h->ctrl, slot, h->DB, h->count_all, (const int*)h->plan);
@@ -1175,16 +1300,12 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
         p += ntiles;
         h->plan = reinterpret_cast<int*>(p);
         if (hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&h->aux2, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&h->ev_join2, hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
             // the handle was value-initialised: what was not created is null
             if (h->aux) (void)hipStreamDestroy(h->aux);
-            if (h->aux2) (void)hipStreamDestroy(h->aux2);
             if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
             if (h->ev_join) (void)hipEventDestroy(h->ev_join);
-            if (h->ev_join2) (void)hipEventDestroy(h->ev_join2);
             (void)hipFree(h->defer_slab);
             (void)hipFree(h->slab);
             delete h;
@@ -1199,10 +1320,8 @@ int hfv_mh2d_create(const hfv_mh2d_config* cfg, hfv_mh2d** out)
     if (hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess) {
         if (h->aux) {
             (void)hipStreamDestroy(h->aux);
-            (void)hipStreamDestroy(h->aux2);
             (void)hipEventDestroy(h->ev_fork);
             (void)hipEventDestroy(h->ev_join);
-            (void)hipEventDestroy(h->ev_join2);
         }
         if (h->defer_slab) (void)hipFree(h->defer_slab);
         (void)hipFree(h->slab);
@@ -1221,12 +1340,9 @@ void hfv_mh2d_destroy(hfv_mh2d* h)
     (void)hipStreamDestroy(h->own);
     if (h->aux) {
         (void)hipStreamSynchronize(h->aux);
-        (void)hipStreamSynchronize(h->aux2);
         (void)hipStreamDestroy(h->aux);
-        (void)hipStreamDestroy(h->aux2);
         (void)hipEventDestroy(h->ev_fork);
         (void)hipEventDestroy(h->ev_join);
-        (void)hipEventDestroy(h->ev_join2);
     }
     if (h->stage) (void)hipFree(h->stage);
     if (h->defer_slab) (void)hipFree(h->defer_slab);
@@ -1411,7 +1527,7 @@ int hfv_mh2d_set_device_bc(hfv_mh2d* h, int side, hfv_device_bc_2d_func fn, void
 int hfv_probe_exact_finish(int impl, double gamma, double tol, int cap, long long n, const double* wl,
                            const double* wr, double* p, int* rc)
 {
-    if ((impl != 0 && impl != 1) || cap < 0 || cap > 100 || n < 0) return HFV_EINVAL;
+    if (impl < 0 || impl > 2 || cap < 0 || cap > 100 || n < 0) return HFV_EINVAL;
     if (n == 0) return HFV_OK;
     double *dl = nullptr, *dr = nullptr, *dp = nullptr;
     int* dc = nullptr;
@@ -1424,7 +1540,10 @@ int hfv_probe_exact_finish(int impl, double gamma, double tol, int cap, long lon
                hipMemcpy(dr, wr, (size_t)n * 4 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
         ret = HFV_EDEVICE;
     } else {
-        if (impl == 1)
+        if (impl == 2)
+            hipLaunchKernelGGL(k_probe_resume_quad, dim3((unsigned)((n + SLOW_FACES_PER_WAVE - 1) / SLOW_FACES_PER_WAVE)),
+                               dim3(LANES), 0, 0, gamma, tol, cap, n, dl, dr, dp, dc);
+        else if (impl == 1)
             hipLaunchKernelGGL(k_probe_finish<true>, dim3((unsigned)n), dim3(LANES), 0, 0, gamma, tol, cap, n, dl, dr,
                                dp, dc);
         else

@@ -184,8 +184,10 @@ def probe_div(lib, impl: int, a: np.ndarray, b: np.ndarray):
 def probe_exact_finish(lib, impl: int, gamma: float, tol: float, cap: int, wl: np.ndarray, wr: np.ndarray):
     """The exact solver's pressure iteration: Newton for cap iterations, then
     the finish of a face not converged, serially (impl 0) or by a wavefront
-    per face (impl 1).  Returns (p*, rc); rc -1: converged within cap, -3:
-    vacuum, else the failure kind."""
+    per face (impl 1); impl 2: Newton for cap iterations, the rest of the
+    loop by four lanes per face (the 2D step's set-aside faces), then the
+    finish.  Returns (p*, rc); rc -1: converged, -3: vacuum, else the
+    failure kind."""
     wl = np.ascontiguousarray(wl, dtype=np.float64)
     wr = np.ascontiguousarray(wr, dtype=np.float64)
     n = wl.shape[0]

@@ -295,9 +295,11 @@ int hfv_probe_limit(int limiter, long long n, const double *a, const double *b, 
  * states wl[n][4], wr[n][4] (rho, u, v, p): Newton from the guess for cap (0-100) iterations,
  * then, for a face not converged, its finish (bracket scan and bisection) as the kernels take
  * it: impl 0 serially per face, impl 1 by a whole wavefront per face (the set-aside faces of
- * the 2D step).  p[n] = p*, rc[n] = the failure kind (HFV_ERR_*), -1 for a face that
- * converged within cap iterations (p: its iterate), -3 for a vacuum face.  impl 1 must equal
- * impl 0 bit for bit. */
+ * the 2D step).  impl 2: Newton for cap iterations serially, the rest of the loop (to 100
+ * iterations) by four lanes per face as the 2D step's set-aside faces take it, then the finish
+ * serially.  p[n] = p*, rc[n] = the failure kind (HFV_ERR_*), -1 for a face that converged
+ * (p: its iterate), -3 for a vacuum face.  impl 1 must equal impl 0 bit for bit at the same
+ * cap, impl 2 must equal impl 0 at cap 100. */
 int hfv_probe_exact_finish(int impl, double gamma, double tol, int cap, long long n, const double *wl,
                            const double *wr, double *p, int *rc);
 

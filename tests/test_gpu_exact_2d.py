@@ -145,3 +145,26 @@ def test_wave_finish_is_the_serial_finish(gpulib, cap, tol):
     assert finished.sum() > 100
     if tol == 0.0:
         assert (rc0 == 4).sum() > 100
+
+
+@pytest.mark.parametrize("tol", [1e-8, 1e-6, 0.0])
+@pytest.mark.parametrize("cap", [0, 1, 8, 50])
+def test_quad_newton_is_the_serial_loop(gpulib, cap, tol):
+    """The set-aside faces' Newton loop by four lanes per face (k_solve_slow's
+    newton_resume_quad: f_K and f'_K of each side on lanes of their own),
+    resumed after cap serial iterations and followed by the serial finish
+    where its outcome says so, against the serial solve (Newton to 100
+    iterations, then the finish): the same p* and failure kind bit for bit,
+    over faces that converge early, late and never (then through the bracket
+    scan and bisection, or their failures)."""
+    from hydrofv_amd import gpu
+    wl, wr = _finish_states()
+    p0, rc0 = gpu.probe_exact_finish(gpulib, 0, 1.4, tol, 100, wl, wr)
+    p2, rc2 = gpu.probe_exact_finish(gpulib, 2, 1.4, tol, cap, wl, wr)
+    assert np.array_equal(rc0, rc2), np.flatnonzero(rc0 != rc2)[:10]
+    same = (p0.view(np.uint64) == p2.view(np.uint64))
+    assert same.all(), [(i, p0[i], p2[i], rc0[i]) for i in np.flatnonzero(~same)[:5]]
+    print(f"cap {cap} tol {tol}: converged {(rc0 == -1).sum()}, finished {(rc0 >= 0).sum()}")
+    assert (rc0 >= 0).sum() > 100
+    if tol > 0.0:
+        assert (rc0 == -1).sum() > 1000
