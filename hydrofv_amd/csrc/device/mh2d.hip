@@ -338,18 +338,22 @@ __device__ PStar newton_finish_wave(const PressureFn& F, double tol, const Newto
 // at ps, by the expression pressure_fn_df evaluates for it (shock_f_df,
 // rarefaction_f_df; exact.c:512-565), computed alone: y is the value's
 // rarefaction exponent (F.zf for f_K, F.zd for f'_K), and pow_pos takes the
-// logarithm that rarefaction_f_df shares between its two powers.
+// logarithm that rarefaction_f_df shares between its two powers.  Both
+// branches' values are computed and the side's branch selects one, so that
+// the shock value's chain runs beside the power's.
 __device__ __forceinline__ double wave_fn_part(const PressureFn& F, const WaveSide& S, double y, bool df, double ps)
 {
-    if (ps > S.p) {
-        const double s = sqrt(S.A / (ps + S.B));
-        const double d = ps - S.p;
-        return df ? (1.0 - 0.5 * d / (S.B - ps)) * s : d * s;
-    }
-    const double r = pow_pos(ps / S.p, y, F.lt, F.et);
+    const double x = ps / S.p;
+    const DD l = pow_log(x, F.lt);
+    const double s = sqrt(S.A / (ps + S.B));
+    const double d = ps - S.p;
+    double vs = df ? (1.0 - 0.5 * d / (S.B - ps)) * s : d * s;
+    asm volatile("" : "+v"(vs));
+    const double r = pow_with_log(x, y, l, F.et);
     // r / p a, or 2 a (r - 1) / (gamma - 1): one division either way
     const double num = df ? r : S.a2 * (r - 1.0);
-    return num / (df ? S.pa : F.gm1);
+    const double vr = num / (df ? S.pa : F.gm1);
+    return (ps > S.p) ? vs : vr;
 }
 
 // v of lane K of the calling lane's quad (lanes 4q .. 4q + 3), by a DPP
@@ -380,13 +384,17 @@ __device__ int newton_resume_quad(const PressureFn& F, double tol, NewtonState s
     const WaveSide S = (role & 2) ? F.s1 : F.s0;
     const double y = df ? F.zd : F.zf;
     const int br0 = st.bracketed;
+    // v: this lane's value at the iterate st.p0, evaluated an iteration
+    // ahead (at the end of the previous one), so that each iterate's
+    // convergence test runs beside the evaluation at the next; iteration 0
+    // takes the stored f and f' at the guess
+    double v = (st.it > 0 && st.it < 100) ? wave_fn_part(F, S, y, df, st.p0) : 0.0;
     for (; st.it < 100; st.it++) {
         double f, d;
         if (st.it == 0) {
             f = st.f_hi;
             d = st.df0;
         } else {
-            const double v = wave_fn_part(F, S, y, df, st.p0);
             const double v0 = quad_lane<0>(v), v1 = quad_lane<1>(v);
             const double v2 = quad_lane<2>(v), v3 = quad_lane<3>(v);
             f = v0 + v2 + F.du;
@@ -399,7 +407,9 @@ __device__ int newton_resume_quad(const PressureFn& F, double tol, NewtonState s
         }
         const double p1 = st.p0 - f / d;
         if (p1 < 0.0) break;
-        if (2.0 * fabs(p1 - st.p0) / (p1 + st.p0) < tol) {
+        const bool conv = 2.0 * fabs(p1 - st.p0) / (p1 + st.p0) < tol;
+        v = wave_fn_part(F, S, y, df, p1);  // the next iteration's (unused if p1 is the last)
+        if (conv) {
             *P = PStar{p1, ERR_NONE};
             return 2;
         }
