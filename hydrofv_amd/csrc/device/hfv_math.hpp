@@ -44,14 +44,20 @@ enum ErrKind : int {
 template <int LIM>
 HFV_HD __forceinline__ double limit(double a, double b)
 {
-    if (a * b <= 0.0) return 0.0;
     if constexpr (LIM == MINMOD) {
+        if (a * b <= 0.0) return 0.0;
         return (fabs(a) < fabs(b)) ? a : b;
     } else if constexpr (LIM == VAN_LEER) {
+        if (a * b <= 0.0) return 0.0;
         return 2.0 * a * b / (a + b + 1e-10);
     } else {
-        if (a > 0) return fmin(0.5 * (a + b), 2.0 * fmin(fabs(a), fabs(b)));
-        return -fmin(-0.5 * (a + b), 2.0 * fmin(fabs(a), fabs(b)));
+        // Where a b > 0, a + b has a's sign and -0.5 (a + b) = 0.5 |a + b|
+        // exactly when a < 0, so both branches of slope_limiter.c:83-90 are
+        // a's sign on fmin(0.5 |a + b|, 2 fmin(|a|, |b|)): computed once, and
+        // selected (no branch) with the a b <= 0 case's zero (:79-82)
+        const double m = fmin(0.5 * fabs(a + b), 2.0 * fmin(fabs(a), fabs(b)));
+        const double r = (a > 0) ? m : -m;
+        return (a * b <= 0.0) ? 0.0 : r;
     }
 }
 
