@@ -452,6 +452,26 @@ HFV_HD __forceinline__ void hllc_guess_begin(double gamma, HllcFace (&F)[N], Hll
         a2[2 * i + 1] = gamma * F[i].wr[3] / F[i].wr[0];
     }
     sqrt_rn<2 * N>(a2);
+    // hllc_is_vacuum's (a_L + a_R) 2 / (gamma + 1), its divisions sharing the
+    // reciprocal of gamma + 1 when every lane's operands allow (div_core),
+    // and evaluated for every face: the test below takes no branch
+    double cv[N];
+    bool cv_ok = div_core_ok(gamma + 1.0);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const double al = (F[i].wl[0] > 0.0) ? a2[2 * i] : 0.0;
+        const double ar = (F[i].wr[0] > 0.0) ? a2[2 * i + 1] : 0.0;
+        cv[i] = (al + ar) * 2.0;
+        cv_ok = cv_ok & div_core_ok(cv[i]);
+    }
+    if (wave_all(cv_ok)) {
+        const double rg = rcp_core(gamma + 1.0);
+#pragma unroll
+        for (int i = 0; i < N; i++) cv[i] = div_core(cv[i], gamma + 1.0, rg);
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; i++) cv[i] = cv[i] / (gamma + 1.0);
+    }
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const double rl = F[i].wl[0], ul = F[i].wl[1], pl = F[i].wl[3];
@@ -460,7 +480,7 @@ HFV_HD __forceinline__ void hllc_guess_begin(double gamma, HllcFace (&F)[N], Hll
         const double ar = (rr > 0.0) ? a2[2 * i + 1] : 0.0;
         F[i].al = al;
         F[i].ar = ar;
-        G.vac[i] = hllc_is_vacuum(gamma, rl, ul, al, rr, ur, ar);
+        G.vac[i] = (rl <= 0.0) | (rr <= 0.0) | (cv[i] <= (ur - ul));  // hllc_is_vacuum
         // riemann_solver.c:154-224 (as guess_select)
         const double plo = (pl > pr) ? pr : pl;
         const double phi = (pl > pr) ? pl : pr;

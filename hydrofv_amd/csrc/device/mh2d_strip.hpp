@@ -695,6 +695,7 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
     const bool out_col = (lane >= HALO) && (lane < HALO + STRIP_COLS) && (col < g + G.nx);
     const bool ghost_lo_col = (sx == 0) && (col == g - 1);               // x_min face flux only
     const bool ghost_hi_col = (sx == G.nsx - 1) && (col == g + G.nx);    // x_max face flux only
+    const bool edge_strip = (sx == 0) || (sx == G.nsx - 1);              // holds a ghost column's lane
     const double gamma = G.gamma, tol = G.tol;
     // DEFER: the lane's x face (left of col) is used by this wavefront when
     // its own cell or the cell on its left is updated here, and queued by
@@ -873,8 +874,10 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             ey = yflux<SOLVER>(gamma, tol, yt, ylo, fyt);
             ex = flux<SOLVER, 4>(gamma, tol, hl, xlo, fxl);
         }
-        if (ey && out_col) { err = ey; err_cell = (long long)r * G.tx + col; }
-        if (ex && (out_col || ghost_hi_col) && lane >= 1) { err = ex; err_cell = (long long)r * G.tx + col - 1; }
+        if (__ballot((ey | ex) != 0) != 0ull) {  // a failed face: rare, tested for the wavefront at once
+            if (ey && out_col) { err = ey; err_cell = (long long)r * G.tx + col; }
+            if (ex && (out_col || ghost_hi_col) && lane >= 1) { err = ex; err_cell = (long long)r * G.tx + col - 1; }
+        }
         double fxr[4];
 #pragma unroll
         for (int f = 0; f < 4; f++) fxr[f] = shfl_from_right(fxl[f]);
@@ -931,12 +934,17 @@ __device__ __forceinline__ StripResult strip_step(const Grid2D& G, double dt, co
             const double sy_ = a + fabs(w[2]);
             if (sx_ > smx) smx = sx_;
             if (sy_ > smy) smy = sy_;
-        } else if (ghost_lo_col) {
+        }
+        // the x ghost columns next to the grid's edges (none of a strip away
+        // from them: the test is the wavefront's, not a lane's)
+        if (edge_strip) {
+            if (!deferred && ghost_lo_col) {
 #pragma unroll
-            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] - fxr[f] * dt * G.area_x;
-        } else if (ghost_hi_col) {
+                for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] - fxr[f] * dt * G.area_x;
+            } else if (!deferred && ghost_hi_col) {
 #pragma unroll
-            for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] + fxl[f] * dt * G.area_x;
+                for (int f = 0; f < 4; f++) V.f[f][k] = U.f[f][k] + fxl[f] * dt * G.area_x;
+            }
         }
 
 #pragma unroll
