@@ -472,6 +472,40 @@ HFV_HD __forceinline__ void hllc_guess_begin(double gamma, HllcFace (&F)[N], Hll
 #pragma unroll
         for (int i = 0; i < N; i++) cv[i] = cv[i] / (gamma + 1.0);
     }
+    // The guess's test p_max / p_min <= 2 (riemann_solver.c:185).  The
+    // quotient rounds to at most 2 exactly when p_max <= (2 + 2^-52) p_min
+    // (2 + 2^-52, the midpoint to the next double, rounds to the even 2), and
+    // for 2^-970 <= p_min, p_max <= 2^1020 the test p_max - 2 p_min <= 2^-52
+    // p_min is that comparison without rounding: 2 p_min and 2^-52 p_min are
+    // exact, and so is the difference where p_min <= p_max <= 4 p_min
+    // (Sterbenz), which holds whenever it can be small enough.  A wavefront
+    // with any other pair divides.
+    bool ratio2[N];
+    bool ratio_ok = true;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const double pl = F[i].wl[3], pr = F[i].wr[3];
+        const double plo = (pl > pr) ? pr : pl;
+        const double phi = (pl > pr) ? pl : pr;
+        ratio_ok = ratio_ok & (plo >= 0x1p-970) & (phi <= 0x1p1020);
+    }
+    if (wave_all(ratio_ok)) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const double pl = F[i].wl[3], pr = F[i].wr[3];
+            const double plo = (pl > pr) ? pr : pl;
+            const double phi = (pl > pr) ? pl : pr;
+            ratio2[i] = (phi - 2.0 * plo) <= 0x1p-52 * plo;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const double pl = F[i].wl[3], pr = F[i].wr[3];
+            const double plo = (pl > pr) ? pr : pl;
+            const double phi = (pl > pr) ? pl : pr;
+            ratio2[i] = phi / plo <= 2.0;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const double rl = F[i].wl[0], ul = F[i].wl[1], pl = F[i].wl[3];
@@ -485,7 +519,7 @@ HFV_HD __forceinline__ void hllc_guess_begin(double gamma, HllcFace (&F)[N], Hll
         const double plo = (pl > pr) ? pr : pl;
         const double phi = (pl > pr) ? pl : pr;
         G.ppv[i] = 0.5 * (pl + pr) - 0.125 * (ur - ul) * (rl + rr) * (al + ar);
-        G.pvrs[i] = (phi / plo <= 2.0) & (plo <= G.ppv[i]) & (G.ppv[i] <= phi);
+        G.pvrs[i] = ratio2[i] & (plo <= G.ppv[i]) & (G.ppv[i] <= phi);
         G.trrs[i] = !G.pvrs[i] & (G.ppv[i] < plo);
         G.num[i] = al + ar - h * (ur - ul);
         G.tsrs[i] = 0.0;

@@ -135,6 +135,36 @@ def test_split_hllc_equals_plain(hc, settle_direct):
     assert np.array_equal(f1[ok], f2[ok], equal_nan=True)
 
 
+@pytest.mark.parametrize("plo_range", [(0.5, 2.0), (2.0 ** -970, 2.0 ** -960), (2.0 ** -975, 2.0 ** -965),
+                                       (2.0 ** 1010, 2.0 ** 1019), (2.0 ** 1015, 2.0 ** 1019.5)])
+def test_split_hllc_pressure_ratio_two(hc, plo_range):
+    """The split HLLC's guess takes p_max / p_min <= 2 without the division
+    (p_max - 2 p_min <= 2^-52 p_min, for 2^-970 <= p_min and p_max <= 2^1020;
+    the host build applies it per face): at ratios within four ulps of 2, in
+    and out of that range, the split flux equals flux_hllc<4> and the oracle
+    bit for bit, and the ratios fall on both sides of the test."""
+    rng = np.random.default_rng(21)
+    n = 20000
+    plo = np.exp(rng.uniform(np.log(plo_range[0]), np.log(plo_range[1]), n))
+    phi = ((2.0 * plo).view(np.int64) + rng.integers(-4, 5, n)).view(np.float64)
+    swap = rng.random(n) < 0.5
+    u = rng.uniform(-0.3, 0.3, n)  # u_L = u_R: the PVRS value lies between the pressures
+    wl = np.stack([rng.uniform(0.5, 2.0, n), u, rng.uniform(-0.2, 0.2, n), np.where(swap, phi, plo)], axis=1)
+    wr = np.stack([rng.uniform(0.5, 2.0, n), u, rng.uniform(-0.2, 0.2, n), np.where(swap, plo, phi)], axis=1)
+    f1, f2 = np.empty((n, 4)), np.empty((n, 4))
+    e1, e2 = np.empty(n, np.int32), np.empty(n, np.int32)
+    hc.hc_flux_hllc(4, GAMMA, TOL, n, wl, wr, f1, e1)
+    hc.hc_flux_hllc_split(GAMMA, TOL, n, wl, wr, f2, e2, 1)
+    fo, eo = O.flux_batch(2, "riemann_solver_hllc", GAMMA, TOL, wl, wr)
+    assert np.array_equal(e1, e2) and np.array_equal(e1 != 0, eo != 0)
+    ok = e1 == 0
+    assert ok.sum() > n // 2
+    assert np.array_equal(f1[ok], f2[ok], equal_nan=True)
+    assert np.array_equal(f2[ok], fo[ok], equal_nan=True)
+    within = phi / plo <= 2.0
+    assert 0.3 * n < within.sum() < 0.9 * n
+
+
 def test_split_hllc_settles_exactly(hc):
     """Two-rarefaction faces on both sides of trrs_settle's test: the split
     form equals the plain flux_hllc bit for bit, and both equal the oracle's

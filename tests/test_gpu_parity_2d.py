@@ -74,6 +74,45 @@ def test_probe_fluxes(gpulib, nf, solver):
         assert pow_free.sum() > 1000
 
 
+def _ratio_two_states(seed, n, plo_range):
+    """Face states whose pressure ratio p_max / p_min lies within a few ulps
+    of 2 (both orientations), with u_L = u_R so that the PVRS value lies
+    between the pressures and the ratio test alone picks the guess."""
+    rng = np.random.default_rng(seed)
+    plo = np.exp(rng.uniform(np.log(plo_range[0]), np.log(plo_range[1]), n))
+    k = rng.integers(-4, 5, n)
+    phi = ((2.0 * plo).view(np.int64) + k).view(np.float64)
+    swap = rng.random(n) < 0.5
+    pl, pr = np.where(swap, phi, plo), np.where(swap, plo, phi)
+    rl, rr = rng.uniform(0.5, 2.0, n), rng.uniform(0.5, 2.0, n)
+    u = rng.uniform(-0.3, 0.3, n)
+    vl, vr = rng.uniform(-0.2, 0.2, n), rng.uniform(-0.2, 0.2, n)
+    return np.stack([rl, u, vl, pl], axis=1), np.stack([rr, u, vr, pr], axis=1)
+
+
+@pytest.mark.parametrize("plo_range", [(0.5, 2.0), (2.0 ** -970, 2.0 ** -960), (2.0 ** -975, 2.0 ** -965),
+                                       (2.0 ** 1010, 2.0 ** 1019), (2.0 ** 1015, 2.0 ** 1019.5)])
+def test_probe_hllc_pressure_ratio_two(gpulib, plo_range):
+    """The 2D kernels' HLLC guess takes p_max / p_min <= 2 as
+    p_max - 2 p_min <= 2^-52 p_min (every pair of a wavefront with
+    2^-970 <= p_min and p_max <= 2^1020; otherwise it divides): pressure
+    ratios within four ulps of 2, in the middle of that range, near its
+    bounds and past them, give flux_hllc's and the oracle's bits."""
+    from hydrofv_amd import gpu
+    import _oracle as O
+    wl, wr = _ratio_two_states(3, 4096, plo_range)
+    fd, ed = gpu.probe_flux(gpulib, 4, 2, 1.4, 1e-6, wl, wr)
+    fo, eo = O.flux_batch(2, "riemann_solver_hllc", 1.4, 1e-6, wl, wr)
+    ed_code = np.select([ed == 0, ed == 2], [0, 2], default=-1)
+    assert np.array_equal(ed_code, eo), [(i, ed[i], eo[i]) for i in np.flatnonzero(ed_code != eo)[:5]]
+    eq = np.all((fd.view(np.uint64) == fo.view(np.uint64)) | (np.isnan(fd) & np.isnan(fo)), axis=1)
+    bad = np.flatnonzero(~eq)
+    assert bad.size == 0, [(wl[i].tolist(), wr[i].tolist(), fd[i].tolist(), fo[i].tolist()) for i in bad[:3]]
+    plo, phi = np.minimum(wl[:, 3], wr[:, 3]), np.maximum(wl[:, 3], wr[:, 3])
+    within = phi / plo <= 2.0
+    assert 500 < within.sum() < 3600, within.sum()
+
+
 def compare(tag, Wg, Ug, Wo, Uo, bitwise=False, tol=TOL, mask=None):
     """Every field within `tol` (relative to the field's largest magnitude);
     bitwise when requested.  Returns whether all fields are bitwise equal."""
