@@ -348,6 +348,9 @@ __device__ __forceinline__ double wave_fn_part(const PressureFn& F, const WaveSi
     const double s = sqrt(S.A / (ps + S.B));
     const double d = ps - S.p;
     double vs = df ? (1.0 - 0.5 * d / (S.B - ps)) * s : d * s;
+    // an empty statement that takes vs: the compiler keeps the shock
+    // value's computation here, ahead of the power's special-case branch,
+    // where it can interleave with the power, instead of sinking it below
     asm volatile("" : "+v"(vs));
     const double r = pow_with_log(x, y, l, F.et);
     // r / p a, or 2 a (r - 1) / (gamma - 1): one division either way
